@@ -1,0 +1,304 @@
+"""aemh -- Python binding of the MI355X MLS-MPM C ABI (include/aemh.h).
+
+A thin ctypes layer over ``libaemh.so`` for tests and the benchmark.  It
+mirrors the reference step API (ae_particle/src/mpm_solver.rs:1063-1084):
+an ``Engine`` owns a device-resident MpssBuffer-shaped particle set, a grid,
+a material table and a config, and ``mpm_step_parallel_with_grid`` advances
+the particles listed in ``particle_indices`` by ``total_dt``.
+
+There is no CPU fallback: importing works anywhere, but creating an Engine
+without the built library or without a gfx950 GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libaemh.so")
+
+# aemh_status
+OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_STATE, ERR_OVERFLOW, ERR_NO_DEVICE = range(7)
+# ConstitutiveModel order (ae_particle/src/constitutive.rs:11-27) + SNOW
+NEO_HOOKEAN, DRUCKER_PRAGER, VON_MISES, NEWTONIAN_FLUID, COROTATED, FIXED_COROTATED, RIGID, SNOW = range(8)
+ACCUM_F32, ACCUM_Q32_32 = 0, 1
+CLAMP_AUTO, CLAMP_SERIAL, CLAMP_WINDOW = 0, 1, 2
+STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
+
+
+class AemhError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"aemh status {status}: {msg}")
+        self.status = status
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("substeps", C.c_uint32),
+        ("gravity", C.c_float * 3),
+        ("damping", C.c_float),
+        ("bc_margin", C.c_uint32),
+        ("bc_restitution", C.c_float),
+        ("accum", C.c_uint32),
+        ("clamp_mode", C.c_uint32),
+        ("storage", C.c_uint32),
+        ("reserved", C.c_uint32 * 4),
+    ]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("first", C.c_uint32),
+        ("count", C.c_uint32),
+        ("nx", C.c_uint32),
+        ("ny", C.c_uint32),
+        ("nz", C.c_uint32),
+        ("dx", C.c_float),
+        ("origin", C.c_float * 3),
+        ("dt", C.c_float),
+        ("velocity_only", C.c_uint32),
+        ("clamp_mode", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+# Every symbol include/aemh.h declares; tests check the library exports them.
+EXPORTS = [
+    "aemh_config_default", "aemh_version", "aemh_create", "aemh_destroy", "aemh_last_error",
+    "aemh_set_config", "aemh_set_grid", "aemh_set_materials", "aemh_upload_particles",
+    "aemh_upload_plastic_j", "aemh_download_particles", "aemh_download_plastic_j",
+    "aemh_set_indices", "aemh_step", "aemh_mpm_step_parallel_with_grid",
+    "aemh_step_velocity_only", "aemh_set_scenes", "aemh_batch_step", "aemh_autotune",
+    "aemh_download_grid", "aemh_download_grid_q32", "aemh_conservation_f64",
+    "aemh_profile_enable", "aemh_profile_get", "aemh_profile_reset", "aemh_scene_uniform_box",
+]
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libaemh.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libaemh.so not built at {path}; run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    fp = C.POINTER(C.c_float)
+    u32 = C.c_uint32
+    lib.aemh_version.restype = C.c_char_p
+    lib.aemh_last_error.restype = C.c_char_p
+    lib.aemh_last_error.argtypes = [P]
+    lib.aemh_config_default.argtypes = [C.POINTER(Config)]
+    lib.aemh_config_default.restype = None
+    lib.aemh_create.argtypes = [C.POINTER(P), C.c_int]
+    lib.aemh_destroy.argtypes = [P]
+    lib.aemh_set_config.argtypes = [P, C.POINTER(Config)]
+    lib.aemh_set_grid.argtypes = [P, u32, u32, u32, C.c_float, fp]
+    lib.aemh_set_materials.argtypes = [P, u32, fp, C.POINTER(C.c_uint8), fp]
+    lib.aemh_upload_particles.argtypes = [P, u32, fp, fp, fp, fp, fp, fp, C.POINTER(C.c_uint16)]
+    lib.aemh_upload_plastic_j.argtypes = [P, fp]
+    lib.aemh_download_particles.argtypes = [P, fp, fp, fp, fp, fp]
+    lib.aemh_download_plastic_j.argtypes = [P, fp]
+    lib.aemh_set_indices.argtypes = [P, C.POINTER(u32), u32]
+    lib.aemh_step.argtypes = [P, C.c_float]
+    lib.aemh_mpm_step_parallel_with_grid.argtypes = [P, C.POINTER(u32), u32, C.c_float]
+    lib.aemh_step_velocity_only.argtypes = [P, C.c_float]
+    lib.aemh_set_scenes.argtypes = [P, C.POINTER(Scene), u32]
+    lib.aemh_batch_step.argtypes = [P, u32]
+    lib.aemh_autotune.argtypes = [P]
+    lib.aemh_download_grid.argtypes = [P, u32, fp, fp, fp, fp]
+    lib.aemh_download_grid_q32.argtypes = [P, u32, C.POINTER(C.c_int64)]
+    lib.aemh_conservation_f64.argtypes = [P, C.POINTER(C.c_double)]
+    lib.aemh_profile_enable.argtypes = [P, C.c_int]
+    lib.aemh_profile_get.argtypes = [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+    lib.aemh_profile_reset.argtypes = [P]
+    lib.aemh_scene_uniform_box.argtypes = [C.c_uint64, u32, fp, fp, fp]
+    lib.aemh_scene_uniform_box.restype = None
+    for name in EXPORTS:
+        f = getattr(lib, name)
+        if name not in ("aemh_version", "aemh_last_error", "aemh_config_default",
+                        "aemh_scene_uniform_box"):
+            f.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _f(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _c(a, dtype):
+    return None if a is None else np.ascontiguousarray(a, dtype=dtype)
+
+
+def default_config() -> Config:
+    cfg = Config()
+    load_library().aemh_config_default(C.byref(cfg))
+    return cfg
+
+
+def uniform_box(seed: int, n: int, lo, hi) -> np.ndarray:
+    """Seeded xoshiro256** positions uniform in the box [lo, hi)."""
+    lib = load_library()
+    out = np.empty((n, 3), np.float32)
+    lo_a = np.asarray(lo, np.float32)
+    hi_a = np.asarray(hi, np.float32)
+    lib.aemh_scene_uniform_box(C.c_uint64(seed), n, _f(lo_a), _f(hi_a), _f(out))
+    return out
+
+
+@dataclass
+class Material:
+    """MaterialParams (ae_particle/src/constitutive.rs:31-39) + model + snow params."""
+    young_modulus: float = 1e6
+    poisson_ratio: float = 0.3
+    yield_stress: float = 1e4
+    hardening: float = 0.01
+    density: float = 1000.0
+    friction_angle: float = 35.0
+    cohesion: float = 0.0
+    model: int = NEO_HOOKEAN
+    theta_c: float = 2.5e-2
+    theta_s: float = 7.5e-3
+    xi: float = 10.0
+
+    def params7(self):
+        return [self.young_modulus, self.poisson_ratio, self.yield_stress, self.hardening,
+                self.density, self.friction_angle, self.cohesion]
+
+
+def material_arrays(mats):
+    params = np.array([m.params7() for m in mats], np.float32)
+    models = np.array([m.model for m in mats], np.uint8)
+    ext = np.array([[m.theta_c, m.theta_s, m.xi] for m in mats], np.float32)
+    return params, models, ext
+
+
+class Engine:
+    """One device context: resident particles, grid, materials, config."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load_library()
+        h = C.c_void_p()
+        st = self._lib.aemh_create(C.byref(h), device)
+        if st != OK:
+            raise AemhError(st, "aemh_create failed (no gfx950 device or HIP unusable)")
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.aemh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != OK:
+            raise AemhError(st, self._lib.aemh_last_error(self._h).decode())
+
+    # -- configuration --------------------------------------------------------
+    def set_config(self, cfg: Config):
+        self._check(self._lib.aemh_set_config(self._h, C.byref(cfg)))
+
+    def set_grid(self, nx, ny, nz, dx, origin=(0.0, 0.0, 0.0)):
+        o = np.asarray(origin, np.float32)
+        self._check(self._lib.aemh_set_grid(self._h, nx, ny, nz, C.c_float(dx), _f(o)))
+
+    def set_materials(self, mats):
+        params, models, ext = material_arrays(mats)
+        self._check(self._lib.aemh_set_materials(
+            self._h, len(mats), _f(params), models.ctypes.data_as(C.POINTER(C.c_uint8)), _f(ext)))
+
+    # -- particles ------------------------------------------------------------
+    def upload(self, pos, vel=None, C9=None, F9=None, J=None, mass=None, mat=None, Jp=None):
+        pos = _c(pos, np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        vel, C9, F9, J, mass = (_c(a, np.float32) for a in (vel, C9, F9, J, mass))
+        mat = _c(mat, np.uint16)
+        self._check(self._lib.aemh_upload_particles(
+            self._h, n, _f(pos), _f(vel), _f(C9), _f(F9), _f(J), _f(mass),
+            None if mat is None else mat.ctypes.data_as(C.POINTER(C.c_uint16))))
+        if Jp is not None:
+            Jp = _c(Jp, np.float32)
+            self._check(self._lib.aemh_upload_plastic_j(self._h, _f(Jp)))
+        self.n = n
+
+    def download(self):
+        n = self.n
+        pos = np.empty((n, 3), np.float32)
+        vel = np.empty((n, 3), np.float32)
+        C9 = np.empty((n, 9), np.float32)
+        F9 = np.empty((n, 9), np.float32)
+        J = np.empty(n, np.float32)
+        self._check(self._lib.aemh_download_particles(self._h, _f(pos), _f(vel), _f(C9), _f(F9), _f(J)))
+        Jp = np.empty(n, np.float32)
+        self._check(self._lib.aemh_download_plastic_j(self._h, _f(Jp)))
+        return dict(pos=pos, vel=vel, C=C9, F=F9, J=J, Jp=Jp)
+
+    def set_indices(self, idx):
+        idx = _c(idx, np.uint32)
+        self._check(self._lib.aemh_set_indices(self._h, idx.ctypes.data_as(C.POINTER(C.c_uint32)), idx.size))
+
+    # -- steps ----------------------------------------------------------------
+    def step(self, total_dt: float):
+        self._check(self._lib.aemh_step(self._h, C.c_float(total_dt)))
+
+    def mpm_step_parallel_with_grid(self, particle_indices, total_dt: float):
+        idx = _c(particle_indices, np.uint32)
+        self._check(self._lib.aemh_mpm_step_parallel_with_grid(
+            self._h, idx.ctypes.data_as(C.POINTER(C.c_uint32)), idx.size, C.c_float(total_dt)))
+
+    def step_velocity_only(self, total_dt: float):
+        self._check(self._lib.aemh_step_velocity_only(self._h, C.c_float(total_dt)))
+
+    def set_scenes(self, scenes):
+        arr = (Scene * len(scenes))(*scenes)
+        self._check(self._lib.aemh_set_scenes(self._h, arr, len(scenes)))
+
+    def batch_step(self, n_substeps: int):
+        self._check(self._lib.aemh_batch_step(self._h, n_substeps))
+
+    def autotune(self):
+        self._check(self._lib.aemh_autotune(self._h))
+
+    # -- readback ---------------------------------------------------------------
+    def download_grid(self, scene=0, nodes=None):
+        m = np.zeros(nodes, np.float32)
+        p = np.zeros((nodes, 3), np.float32)
+        f = np.zeros((nodes, 3), np.float32)
+        v = np.zeros((nodes, 3), np.float32)
+        self._check(self._lib.aemh_download_grid(self._h, scene, _f(m), _f(p), _f(f), _f(v)))
+        return dict(m=m, p=p, f=f, v=v)
+
+    def download_grid_q32(self, scene=0, nodes=None):
+        q = np.zeros((nodes, 7), np.int64)
+        self._check(self._lib.aemh_download_grid_q32(self._h, scene, q.ctypes.data_as(C.POINTER(C.c_int64))))
+        return q
+
+    def conservation_f64(self):
+        out = (C.c_double * 5)()
+        self._check(self._lib.aemh_conservation_f64(self._h, out))
+        return np.array(out[:], np.float64)
+
+    # -- profiling ----------------------------------------------------------------
+    def profile(self, on=True):
+        self._check(self._lib.aemh_profile_enable(self._h, 1 if on else 0))
+
+    def profile_reset(self):
+        self._check(self._lib.aemh_profile_reset(self._h))
+
+    def profile_get(self, kernel: str):
+        ms = C.c_double()
+        cnt = C.c_uint64()
+        self._check(self._lib.aemh_profile_get(self._h, kernel.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value

@@ -1,0 +1,923 @@
+// aemh_api.hip -- host side of the C ABI declared in include/aemh.h.
+//
+// Owns every device buffer of a context: the canonical MpssBuffer-shaped
+// particle arrays (capacity n, original slot order), two working buffers of
+// the active set that G2P ping-pongs between (leaving them sorted by base
+// block), the block-major grid, the sort bookkeeping, and the scene table.
+// A substep is a fixed sequence of kernels on one stream; repeated substeps
+// are captured once into a hipGraph per ping-pong parity and replayed.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <cmath>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/aemh.h"
+#include "aemh_common.h"
+#include "aemh_launch.h"
+
+using namespace aemh;
+
+namespace {
+
+const char* kVersion = "aemh 0.1 (gfx950)";
+
+struct ParticleStore {
+  uint32_t cap = 0;
+  int storage = 0;
+  void* slab = nullptr;
+  PBuf pb{};
+};
+
+struct ProfRec {
+  double total_ms = 0;
+  uint64_t count = 0;
+};
+
+}  // namespace
+
+struct aemh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  aemh_config cfg{};
+  // context grid (single-scene API)
+  uint32_t nx = 0, ny = 0, nz = 0;
+  float dx = 0, origin[3] = {0, 0, 0};
+  bool grid_set = false;
+  // materials (n + 1 entries: the last is the unwrap_or fallback)
+  std::vector<MatDev> mats_h;
+  MatDev* mats_d = nullptr;
+  uint32_t nmat = 0;
+  // particles
+  ParticleStore canon;
+  ParticleStore work[2];
+  int cur = 0;
+  uint32_t n_work = 0;
+  bool work_valid = false, work_dirty = false;
+  std::vector<uint32_t> idx_h;
+  uint32_t* idx_d = nullptr;
+  float* staging = nullptr;
+  size_t staging_bytes = 0;
+  // scenes
+  std::vector<aemh_scene> scenes_h;  // batch table as given
+  bool batch_mode = false;
+  std::vector<SceneDev> scd_h;
+  SceneDev* scd_d = nullptr;
+  size_t scd_cap = 0;
+  uint32_t nb_total = 0;
+  // grid + sort
+  GridDev grid{};
+  int grid_q32 = 0;
+  uint32_t grid_nb_cap = 0;
+  SortDev sort{};
+  uint32_t sort_n_cap = 0, sort_nb_cap = 0;
+  LaunchShape shape;
+  bool tuned = false;
+  // graphs
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  bool graph_valid = false;
+  bool use_graph = true;
+  // profiling
+  bool prof = false;
+  std::map<std::string, ProfRec> prof_rec;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> ev_pool;
+  // pinned readback word for overflow checks
+  unsigned long long* qmax_h = nullptr;
+};
+
+namespace {
+
+aemh_status fail(aemh_ctx* c, aemh_status s, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return s;
+}
+
+#define HIPTRY(ctx, expr)                                                                    \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) {                                                                  \
+      return fail(ctx, e_ == hipErrorOutOfMemory ? AEMH_ERR_OOM : AEMH_ERR_HIP, "%s: %s (%s)", \
+                  #expr, hipGetErrorName(e_), hipGetErrorString(e_));                        \
+    }                                                                                        \
+  } while (0)
+
+#define CHECK_LAUNCH(ctx)                                                                    \
+  do {                                                                                       \
+    hipError_t e_ = hipGetLastError();                                                       \
+    if (e_ != hipSuccess)                                                                    \
+      return fail(ctx, AEMH_ERR_HIP, "kernel launch: %s (%s)", hipGetErrorName(e_),          \
+                  hipGetErrorString(e_));                                                    \
+  } while (0)
+
+template <typename T>
+T* carve(char*& p, size_t count) {
+  T* r = reinterpret_cast<T*>(p);
+  size_t bytes = (count * sizeof(T) + 255) & ~(size_t)255;
+  p += bytes;
+  return r;
+}
+
+size_t store_bytes(uint32_t n, int storage) {
+  auto a = [&](size_t elem) { return ((size_t)n * elem + 255) & ~(size_t)255; };
+  size_t b = 6 * a(4);                          // x, v
+  b += storage == 0 ? 18 * a(4) : 18 * a(2);   // C, F
+  b += 3 * a(4);                                // J, Jp, mass
+  b += a(2) + a(4);                             // mat, id
+  return b;
+}
+
+aemh_status alloc_store(aemh_ctx* c, ParticleStore& s, uint32_t n, int storage) {
+  if (s.slab && s.cap >= n && s.storage == storage) return AEMH_OK;
+  if (s.slab) {
+    hipFree(s.slab);
+    s.slab = nullptr;
+  }
+  s.cap = n;
+  s.storage = storage;
+  s.pb = PBuf{};
+  if (n == 0) return AEMH_OK;
+  HIPTRY(c, hipMalloc(&s.slab, store_bytes(n, storage)));
+  char* p = (char*)s.slab;
+  for (int a = 0; a < 3; a++) s.pb.x[a] = carve<float>(p, n);
+  for (int a = 0; a < 3; a++) s.pb.v[a] = carve<float>(p, n);
+  for (int k = 0; k < 9; k++) {
+    if (storage == 0) {
+      s.pb.C[k] = carve<float>(p, n);
+      s.pb.F[k] = carve<float>(p, n);
+    } else {
+      s.pb.Ch[k] = carve<uint16_t>(p, n);
+      s.pb.Fh[k] = carve<uint16_t>(p, n);
+    }
+  }
+  s.pb.J = carve<float>(p, n);
+  s.pb.Jp = carve<float>(p, n);
+  s.pb.mass = carve<float>(p, n);
+  s.pb.mat = carve<uint16_t>(p, n);
+  s.pb.id = carve<uint32_t>(p, n);
+  return AEMH_OK;
+}
+
+aemh_status ensure_staging(aemh_ctx* c, size_t bytes) {
+  if (c->staging_bytes >= bytes) return AEMH_OK;
+  if (c->staging) hipFree(c->staging);
+  c->staging = nullptr;
+  c->staging_bytes = 0;
+  HIPTRY(c, hipMalloc(&c->staging, bytes));
+  c->staging_bytes = bytes;
+  return AEMH_OK;
+}
+
+void invalidate_graphs(aemh_ctx* c) {
+  for (int i = 0; i < 2; i++)
+    if (c->gexec[i]) {
+      hipGraphExecDestroy(c->gexec[i]);
+      c->gexec[i] = nullptr;
+    }
+  c->graph_valid = false;
+}
+
+// ---------------------------------------------------------------------------
+// Material constants with the reference's f32 evaluation order
+// (constitutive.rs:55-82, :485-490); contraction is off in this file.
+// ---------------------------------------------------------------------------
+MatDev make_mat(const float* p, int model, const float* ext) {
+  MatDev m{};
+  const float E = p[0];
+  float nu = p[1];
+  if (nu < -0.999f) nu = -0.999f;
+  if (nu > 0.4999f) nu = 0.4999f;
+  m.lambda = E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
+  m.mu = E / (2.0f * (1.0f + nu));
+  m.bulk = E / (3.0f * (1.0f - 2.0f * nu));
+  m.yield_stress = p[2];
+  m.density = p[4];
+  const float rads_per_deg = 3.14159265358979323846f / 180.0f;
+  const float phi = p[5] * rads_per_deg;
+  const float sphi = sinf(phi), cphi = cosf(phi);
+  const float sqrt3 = sqrtf(3.0f);
+  m.dp_alpha = (2.0f * sphi) / (sqrt3 * (3.0f - sphi));
+  m.dp_k = (6.0f * p[6] * cphi) / (sqrt3 * (3.0f - sphi));
+  m.theta_c = ext ? ext[0] : 2.5e-2f;
+  m.theta_s = ext ? ext[1] : 7.5e-3f;
+  m.xi = ext ? ext[2] : 10.0f;
+  m.model = model;
+  return m;
+}
+
+SceneDev make_scene(const aemh_scene& s, const aemh_config& cfg, uint32_t block_off) {
+  SceneDev d{};
+  d.first = s.first;
+  d.count = s.count;
+  d.n[0] = (int)s.nx;
+  d.n[1] = (int)s.ny;
+  d.n[2] = (int)s.nz;
+  for (int a = 0; a < 3; a++) d.nb[a] = (d.n[a] + 3) / 4;
+  d.block_off = block_off;
+  d.dx = s.dx;
+  d.inv_dx = 1.0f / s.dx;  // Grid::new (mpm_solver.rs:100)
+  for (int a = 0; a < 3; a++) d.o[a] = s.origin[a];
+  d.dt = s.dt;
+  for (int a = 0; a < 3; a++) d.g[a] = cfg.gravity[a];
+  d.damping = cfg.damping;
+  d.bc_margin = (int)cfg.bc_margin;
+  d.bc_rest = cfg.bc_restitution;
+  d.vel_only = s.velocity_only ? 1u : 0u;
+  uint32_t cm = s.clamp_mode;
+  d.window_clamp = (cm == AEMH_CLAMP_WINDOW || (cm == AEMH_CLAMP_AUTO && s.count >= 50000)) ? 1u : 0u;
+  d.gmax_x = (float)(s.nx - 1) * s.dx;
+  d.gmax_z = (float)(s.nz - 1) * s.dx;
+  d.four_inv_dx2 = 4.0f * d.inv_dx * d.inv_dx;
+  return d;
+}
+
+aemh_status ensure_grid_sort(aemh_ctx* c) {
+  const uint32_t nb = c->nb_total;
+  const uint32_t n = c->n_work;
+  const int q32 = c->cfg.accum == AEMH_ACCUM_Q32_32;
+  if (nb > c->grid_nb_cap || q32 != c->grid_q32 || !c->grid.gv) {
+    if (c->grid.accf) hipFree(c->grid.accf);
+    if (c->grid.accq) hipFree(c->grid.accq);
+    if (c->grid.gv) hipFree(c->grid.gv);
+    c->grid = GridDev{};
+    const size_t nodes = (size_t)nb * kBlockNodes;
+    if (nodes >= (1ull << 32)) return fail(c, AEMH_ERR_ARG, "grid too large: %zu nodes", nodes);
+    if (q32) {
+      HIPTRY(c, hipMalloc(&c->grid.accq, 7 * nodes * sizeof(long long)));
+      HIPTRY(c, hipMemset(c->grid.accq, 0, 7 * nodes * sizeof(long long)));
+    } else {
+      HIPTRY(c, hipMalloc(&c->grid.accf, 7 * nodes * sizeof(float)));
+      HIPTRY(c, hipMemset(c->grid.accf, 0, 7 * nodes * sizeof(float)));
+    }
+    HIPTRY(c, hipMalloc(&c->grid.gv, nodes * sizeof(float4)));
+    HIPTRY(c, hipMemset(c->grid.gv, 0, nodes * sizeof(float4)));
+    c->grid.nodes = (uint32_t)nodes;
+    c->grid_nb_cap = nb;
+    c->grid_q32 = q32;
+    invalidate_graphs(c);
+  }
+  if (nb > c->sort_nb_cap || !c->sort.cnt) {
+    for (void* p : {(void*)c->sort.cnt, (void*)c->sort.off, (void*)c->sort.tsum,
+                    (void*)c->sort.flag, (void*)c->sort.active})
+      if (p) hipFree(p);
+    const uint32_t tiles = (nb + kScanTile - 1) / kScanTile;
+    HIPTRY(c, hipMalloc(&c->sort.cnt, (size_t)nb * 4));
+    HIPTRY(c, hipMemset(c->sort.cnt, 0, (size_t)nb * 4));
+    HIPTRY(c, hipMalloc(&c->sort.off, (size_t)nb * 4));
+    HIPTRY(c, hipMalloc(&c->sort.tsum, (size_t)(tiles + 1) * 4));
+    HIPTRY(c, hipMalloc(&c->sort.flag, (size_t)nb * 4));
+    HIPTRY(c, hipMemset(c->sort.flag, 0, (size_t)nb * 4));
+    HIPTRY(c, hipMalloc(&c->sort.active, (size_t)nb * 4));
+    HIPTRY(c, hipMemset(c->sort.active, 0, (size_t)nb * 4));
+    c->sort_nb_cap = nb;
+    invalidate_graphs(c);
+  }
+  if (!c->sort.counters) {
+    HIPTRY(c, hipMalloc(&c->sort.counters, 64));
+    HIPTRY(c, hipMemset(c->sort.counters, 0, 64));
+    HIPTRY(c, hipMalloc(&c->sort.qmax, 8));
+    HIPTRY(c, hipMemset(c->sort.qmax, 0, 8));
+  }
+  if (n > c->sort_n_cap || !c->sort.key) {
+    for (void* p : {(void*)c->sort.key, (void*)c->sort.rank, (void*)c->sort.perm})
+      if (p) hipFree(p);
+    const size_t b = (size_t)(n ? n : 1) * 4;
+    HIPTRY(c, hipMalloc(&c->sort.key, b));
+    HIPTRY(c, hipMalloc(&c->sort.rank, b));
+    HIPTRY(c, hipMalloc(&c->sort.perm, b));
+    c->sort_n_cap = n;
+    invalidate_graphs(c);
+  }
+  return AEMH_OK;
+}
+
+// Upload the scene table (contents only: graphs read it at replay time).
+aemh_status set_scene_table(aemh_ctx* c, const std::vector<aemh_scene>& sc) {
+  std::vector<SceneDev> d;
+  d.reserve(sc.size());
+  uint32_t off = 0, expect = 0;
+  for (const auto& s : sc) {
+    if (s.first != expect)
+      return fail(c, AEMH_ERR_ARG, "scenes must tile the active set in order (first=%u, expected %u)",
+                  s.first, expect);
+    if (s.nx == 0 || s.ny == 0 || s.nz == 0 || !(s.dx > 0.0f))
+      return fail(c, AEMH_ERR_ARG, "scene grid must be non-empty with dx > 0");
+    expect += s.count;
+    d.push_back(make_scene(s, c->cfg, off));
+    uint64_t nbs = (uint64_t)d.back().nb[0] * d.back().nb[1] * d.back().nb[2];
+    if (off + nbs >= (1ull << 26)) return fail(c, AEMH_ERR_ARG, "too many grid blocks");
+    off += (uint32_t)nbs;
+  }
+  if (expect != c->n_work)
+    return fail(c, AEMH_ERR_ARG, "scenes cover %u particles, active set has %u", expect, c->n_work);
+  const bool shape_changed = d.size() != c->scd_h.size() || off != c->nb_total;
+  c->nb_total = off;
+  if (d.size() > c->scd_cap) {
+    if (c->scd_d) hipFree(c->scd_d);
+    c->scd_d = nullptr;
+    HIPTRY(c, hipMalloc(&c->scd_d, d.size() * sizeof(SceneDev)));
+    c->scd_cap = d.size();
+    invalidate_graphs(c);
+  }
+  if (shape_changed) invalidate_graphs(c);
+  c->scd_h = d;
+  HIPTRY(c, hipMemcpyAsync(c->scd_d, d.data(), d.size() * sizeof(SceneDev), hipMemcpyHostToDevice,
+                           c->stream));
+  return ensure_grid_sort(c);
+}
+
+hipEvent_t get_event(aemh_ctx* c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+template <typename Fn>
+void timed(aemh_ctx* c, const char* name, Fn&& fn) {
+  if (!c->prof) {
+    fn();
+    return;
+  }
+  hipEvent_t a = get_event(c), b = get_event(c);
+  (void)hipEventRecord(a, c->stream);
+  fn();
+  (void)hipEventRecord(b, c->stream);
+  c->pending.push_back({name, {a, b}});
+}
+
+void collect_profile(aemh_ctx* c) {
+  for (auto& pr : c->pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, pr.second.first, pr.second.second) == hipSuccess) {
+      auto& r = c->prof_rec[pr.first];
+      r.total_ms += ms;
+      r.count += 1;
+    }
+    c->ev_pool.push_back(pr.second.first);
+    c->ev_pool.push_back(pr.second.second);
+  }
+  c->pending.clear();
+}
+
+// Launch one substep on the context stream (eager or under capture).
+void enqueue_substep(aemh_ctx* c, int cur) {
+  const PBuf a = c->work[cur].pb, b = c->work[cur ^ 1].pb;
+  const uint32_t n = c->n_work;
+  const int ns = (int)c->scd_h.size();
+  const int storage = c->work[0].storage;
+  const int q32 = c->cfg.accum == AEMH_ACCUM_Q32_32;
+  hipStream_t st = c->stream;
+  timed(c, "step_begin", [&] { launch_step_begin(st, c->sort); });
+  timed(c, "count", [&] { launch_count(st, a, c->scd_d, ns, n, c->sort); });
+  timed(c, "scan", [&] { launch_scan(st, c->sort, c->nb_total); });
+  timed(c, "scatter", [&] { launch_scatter(st, n, c->sort); });
+  timed(c, "clear", [&] { launch_clear(st, c->sort, c->grid, q32, c->shape.upd_wgs); });
+  if (q32) {
+    timed(c, "p2g", [&] {
+      det::launch_p2g(st, c->shape, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
+    });
+    timed(c, "grid_update", [&] { det::launch_grid_update(st, c->shape, c->scd_d, ns, c->sort, c->grid); });
+    timed(c, "g2p", [&] {
+      det::launch_g2p(st, c->shape, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
+    });
+  } else {
+    timed(c, "p2g", [&] {
+      fast::launch_p2g(st, c->shape, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
+    });
+    timed(c, "grid_update", [&] { fast::launch_grid_update(st, c->shape, c->scd_d, ns, c->sort, c->grid); });
+    timed(c, "g2p", [&] {
+      fast::launch_g2p(st, c->shape, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
+    });
+  }
+}
+
+aemh_status build_graphs(aemh_ctx* c) {
+  invalidate_graphs(c);
+  for (int par = 0; par < 2; par++) {
+    hipGraph_t g = nullptr;
+    HIPTRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_substep(c, par);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess) return fail(c, AEMH_ERR_HIP, "graph capture: %s", hipGetErrorName(e));
+    e = hipGraphInstantiate(&c->gexec[par], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(c, AEMH_ERR_HIP, "graph instantiate: %s", hipGetErrorName(e));
+  }
+  c->graph_valid = true;
+  return AEMH_OK;
+}
+
+aemh_status check_overflow(aemh_ctx* c) {
+  if (c->cfg.accum != AEMH_ACCUM_Q32_32) return AEMH_OK;
+  HIPTRY(c, hipMemcpyAsync(c->qmax_h, c->sort.qmax, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  // A node receives at most one contribution per particle and field, so
+  // |sum| <= n * max|contribution| bounds every Q32.32 node sum.
+  unsigned __int128 bound = (unsigned __int128)(*c->qmax_h) * (unsigned __int128)c->n_work;
+  if (bound >= ((unsigned __int128)1 << 63))
+    return fail(c, AEMH_ERR_OVERFLOW,
+                "Q32.32 accumulation may have wrapped (max |contribution| %llu raw, %u particles)",
+                *c->qmax_h, c->n_work);
+  return AEMH_OK;
+}
+
+aemh_status run_substeps(aemh_ctx* c, uint32_t k) {
+  if (!c->mats_d) return fail(c, AEMH_ERR_STATE, "materials not set");
+  if (c->n_work == 0) return AEMH_OK;
+  const bool graph = c->use_graph && !c->prof;
+  if (graph && !c->graph_valid) {
+    aemh_status s = build_graphs(c);
+    if (s != AEMH_OK) return s;
+  }
+  for (uint32_t i = 0; i < k; i++) {
+    if (graph) {
+      HIPTRY(c, hipGraphLaunch(c->gexec[c->cur], c->stream));
+    } else {
+      enqueue_substep(c, c->cur);
+      CHECK_LAUNCH(c);
+    }
+    c->cur ^= 1;
+  }
+  c->work_dirty = true;
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  if (c->prof) collect_profile(c);
+  return check_overflow(c);
+}
+
+aemh_status ensure_work(aemh_ctx* c) {
+  if (c->work_valid) return AEMH_OK;
+  if (!c->canon.slab) return fail(c, AEMH_ERR_STATE, "no particles uploaded");
+  std::vector<uint32_t> all(c->canon.cap);
+  for (uint32_t i = 0; i < c->canon.cap; i++) all[i] = i;
+  return aemh_set_indices(c, all.data(), c->canon.cap);
+}
+
+aemh_status flush_work(aemh_ctx* c) {
+  if (!c->work_valid || !c->work_dirty) return AEMH_OK;
+  launch_scatter_work(c->stream, c->work[c->cur].pb, c->canon.pb, c->idx_d, c->n_work,
+                      c->work[0].storage);
+  CHECK_LAUNCH(c);
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  c->work_dirty = false;
+  return AEMH_OK;
+}
+
+aemh_status single_scene(aemh_ctx* c, float total_dt, int vel_only) {
+  if (!c->grid_set) return fail(c, AEMH_ERR_STATE, "grid not set");
+  if (c->cfg.substeps == 0) return fail(c, AEMH_ERR_ARG, "substeps must be >= 1");
+  aemh_status s = ensure_work(c);
+  if (s != AEMH_OK) return s;
+  aemh_scene sc{};
+  sc.first = 0;
+  sc.count = c->n_work;
+  sc.nx = c->nx;
+  sc.ny = c->ny;
+  sc.nz = c->nz;
+  sc.dx = c->dx;
+  for (int a = 0; a < 3; a++) sc.origin[a] = c->origin[a];
+  sc.dt = total_dt / (float)c->cfg.substeps;  // mpm_solver.rs:1075
+  sc.velocity_only = vel_only;
+  sc.clamp_mode = c->cfg.clamp_mode;
+  c->batch_mode = false;
+  s = set_scene_table(c, {sc});
+  if (s != AEMH_OK) return s;
+  return run_substeps(c, c->cfg.substeps);
+}
+
+struct Ptr9 {
+  float* p[9];
+};
+
+}  // namespace
+
+namespace {
+__global__ void k_aos_to_soa_v(const float* __restrict__ aos, Ptr9 soa, int comps, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < comps; c++) soa.p[c][i] = aos[(size_t)i * comps + c];
+}
+__global__ void k_soa_to_aos_v(Ptr9 soa, float* __restrict__ aos, int comps, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < comps; c++) aos[(size_t)i * comps + c] = soa.p[c][i];
+}
+
+aemh_status upload_aos(aemh_ctx* c, const float* host, float* const* dst, int comps, uint32_t n) {
+  if (!host) return AEMH_OK;
+  aemh_status s = ensure_staging(c, (size_t)n * comps * 4);
+  if (s != AEMH_OK) return s;
+  HIPTRY(c, hipMemcpyAsync(c->staging, host, (size_t)n * comps * 4, hipMemcpyHostToDevice, c->stream));
+  Ptr9 p{};
+  for (int k = 0; k < comps; k++) p.p[k] = dst[k];
+  k_aos_to_soa_v<<<(n + 255) / 256, 256, 0, c->stream>>>(c->staging, p, comps, n);
+  CHECK_LAUNCH(c);
+  return AEMH_OK;
+}
+aemh_status download_aos(aemh_ctx* c, float* host, float* const* src, int comps, uint32_t n) {
+  if (!host) return AEMH_OK;
+  aemh_status s = ensure_staging(c, (size_t)n * comps * 4);
+  if (s != AEMH_OK) return s;
+  Ptr9 p{};
+  for (int k = 0; k < comps; k++) p.p[k] = src[k];
+  k_soa_to_aos_v<<<(n + 255) / 256, 256, 0, c->stream>>>(p, c->staging, comps, n);
+  CHECK_LAUNCH(c);
+  HIPTRY(c, hipMemcpyAsync(host, c->staging, (size_t)n * comps * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  return AEMH_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+void aemh_config_default(aemh_config* cfg) {
+  if (!cfg) return;
+  memset(cfg, 0, sizeof(*cfg));
+  // MpmConfig::default (mpm_solver.rs:57-69)
+  cfg->substeps = 1;
+  cfg->gravity[0] = 0.0f;
+  cfg->gravity[1] = -9.81f;
+  cfg->gravity[2] = 0.0f;
+  cfg->damping = 0.995f;
+  cfg->bc_margin = 0;
+  cfg->bc_restitution = 0.0f;
+  cfg->accum = AEMH_ACCUM_F32;
+  cfg->clamp_mode = AEMH_CLAMP_AUTO;
+  cfg->storage = AEMH_STORE_F32;
+}
+
+const char* aemh_version(void) { return kVersion; }
+
+aemh_status aemh_create(aemh_ctx** out, int device) {
+  if (!out) return AEMH_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return AEMH_ERR_NO_DEVICE;
+  if (device < 0 || device >= count) return AEMH_ERR_ARG;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return AEMH_ERR_NO_DEVICE;
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return AEMH_ERR_NO_DEVICE;
+  aemh_ctx* c = new aemh_ctx();
+  c->device = device;
+  aemh_config_default(&c->cfg);
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+      hipHostMalloc((void**)&c->qmax_h, 8, hipHostMallocDefault) != hipSuccess) {
+    delete c;
+    return AEMH_ERR_HIP;
+  }
+  *out = c;
+  return AEMH_OK;
+}
+
+aemh_status aemh_destroy(aemh_ctx* c) {
+  if (!c) return AEMH_OK;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  invalidate_graphs(c);
+  for (auto& pr : c->pending) {
+    hipEventDestroy(pr.second.first);
+    hipEventDestroy(pr.second.second);
+  }
+  for (auto e : c->ev_pool) hipEventDestroy(e);
+  for (void* p : {(void*)c->mats_d, c->canon.slab, c->work[0].slab, c->work[1].slab,
+                  (void*)c->idx_d, (void*)c->staging, (void*)c->scd_d, (void*)c->grid.accf,
+                  (void*)c->grid.accq, (void*)c->grid.gv, (void*)c->sort.key, (void*)c->sort.rank,
+                  (void*)c->sort.perm, (void*)c->sort.cnt, (void*)c->sort.off, (void*)c->sort.tsum,
+                  (void*)c->sort.flag, (void*)c->sort.active, (void*)c->sort.counters,
+                  (void*)c->sort.qmax})
+    if (p) hipFree(p);
+  if (c->qmax_h) hipHostFree(c->qmax_h);
+  hipStreamDestroy(c->stream);
+  delete c;
+  return AEMH_OK;
+}
+
+const char* aemh_last_error(const aemh_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+aemh_status aemh_set_config(aemh_ctx* c, const aemh_config* cfg) {
+  if (!c || !cfg) return fail(c, AEMH_ERR_ARG, "null argument");
+  if (cfg->substeps == 0) return fail(c, AEMH_ERR_ARG, "substeps must be >= 1");
+  if (cfg->accum > AEMH_ACCUM_Q32_32 || cfg->clamp_mode > AEMH_CLAMP_WINDOW ||
+      cfg->storage > AEMH_STORE_F16)
+    return fail(c, AEMH_ERR_ARG, "unknown mode value");
+  hipSetDevice(c->device);
+  const bool storage_changed = cfg->storage != c->cfg.storage;
+  if (storage_changed && c->work_valid) {
+    aemh_status s = flush_work(c);
+    if (s != AEMH_OK) return s;
+    c->work_valid = false;
+  }
+  if (cfg->accum != c->cfg.accum || storage_changed) invalidate_graphs(c);
+  c->cfg = *cfg;
+  return AEMH_OK;
+}
+
+aemh_status aemh_set_grid(aemh_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, float dx,
+                          const float origin[3]) {
+  if (!c) return AEMH_ERR_ARG;
+  if (nx == 0 || ny == 0 || nz == 0 || !(dx > 0.0f) || !std::isfinite(dx))
+    return fail(c, AEMH_ERR_ARG, "grid dims must be > 0 and dx > 0");
+  if ((uint64_t)nx * ny * nz > (1ull << 31)) return fail(c, AEMH_ERR_ARG, "grid too large");
+  c->nx = nx;
+  c->ny = ny;
+  c->nz = nz;
+  c->dx = dx;
+  for (int a = 0; a < 3; a++) c->origin[a] = origin ? origin[a] : 0.0f;
+  c->grid_set = true;
+  return AEMH_OK;
+}
+
+aemh_status aemh_set_materials(aemh_ctx* c, uint32_t n, const float* params7,
+                               const uint8_t* models, const float* ext3) {
+  if (!c) return AEMH_ERR_ARG;
+  if (n == 0 || !params7 || !models)
+    return fail(c, AEMH_ERR_ARG, "material table must be non-empty (material_table[0] is the fallback)");
+  hipSetDevice(c->device);
+  std::vector<MatDev> m(n + 1);
+  for (uint32_t i = 0; i < n; i++)
+    m[i] = make_mat(params7 + 7 * (size_t)i, models[i], ext3 ? ext3 + 3 * (size_t)i : nullptr);
+  // unwrap_or(&material_table[0]) with NeoHookean (mpm_solver.rs:840-841)
+  m[n] = make_mat(params7, AEMH_MODEL_NEO_HOOKEAN, ext3);
+  if (n + 1 > c->mats_h.size() || !c->mats_d) {
+    if (c->mats_d) hipFree(c->mats_d);
+    c->mats_d = nullptr;
+    HIPTRY(c, hipMalloc(&c->mats_d, (n + 1) * sizeof(MatDev)));
+    invalidate_graphs(c);
+  }
+  if (n != c->nmat) invalidate_graphs(c);
+  c->mats_h = m;
+  c->nmat = n;
+  HIPTRY(c, hipMemcpy(c->mats_d, m.data(), (n + 1) * sizeof(MatDev), hipMemcpyHostToDevice));
+  return AEMH_OK;
+}
+
+aemh_status aemh_upload_particles(aemh_ctx* c, uint32_t n, const float* pos3, const float* vel3,
+                                  const float* C9, const float* F9, const float* J,
+                                  const float* mass, const uint16_t* mat) {
+  if (!c) return AEMH_ERR_ARG;
+  if (n == 0 || !pos3) return fail(c, AEMH_ERR_ARG, "need n > 0 and positions");
+  hipSetDevice(c->device);
+  aemh_status s = alloc_store(c, c->canon, n, 0);
+  if (s != AEMH_OK) return s;
+  c->canon.cap = n;
+  PBuf& P = c->canon.pb;
+  // MpssBuffer::new / reset_slot defaults (mpss.rs:161-185, 215-232)
+  launch_fill(c->stream, P.J, 1.0f, n);
+  launch_fill(c->stream, P.Jp, 1.0f, n);
+  launch_fill(c->stream, P.mass, 1.0f, n);
+  for (int a = 0; a < 3; a++) launch_fill(c->stream, P.v[a], 0.0f, n);
+  for (int k = 0; k < 9; k++) {
+    launch_fill(c->stream, P.C[k], 0.0f, n);
+    launch_fill(c->stream, P.F[k], (k % 4 == 0) ? 1.0f : 0.0f, n);
+  }
+  HIPTRY(c, hipMemsetAsync(P.mat, 0, (size_t)n * 2, c->stream));
+  if ((s = upload_aos(c, pos3, P.x, 3, n)) != AEMH_OK) return s;
+  if ((s = upload_aos(c, vel3, P.v, 3, n)) != AEMH_OK) return s;
+  if ((s = upload_aos(c, C9, P.C, 9, n)) != AEMH_OK) return s;
+  if ((s = upload_aos(c, F9, P.F, 9, n)) != AEMH_OK) return s;
+  if (J) HIPTRY(c, hipMemcpyAsync(P.J, J, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+  if (mass) HIPTRY(c, hipMemcpyAsync(P.mass, mass, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+  if (mat) HIPTRY(c, hipMemcpyAsync(P.mat, mat, (size_t)n * 2, hipMemcpyHostToDevice, c->stream));
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  CHECK_LAUNCH(c);
+  c->work_valid = false;
+  c->work_dirty = false;
+  if (c->sort.qmax) HIPTRY(c, hipMemset(c->sort.qmax, 0, 8));
+  return AEMH_OK;
+}
+
+aemh_status aemh_upload_plastic_j(aemh_ctx* c, const float* Jp) {
+  if (!c || !Jp) return fail(c, AEMH_ERR_ARG, "null argument");
+  if (!c->canon.slab) return fail(c, AEMH_ERR_STATE, "no particles uploaded");
+  hipSetDevice(c->device);
+  aemh_status s = flush_work(c);
+  if (s != AEMH_OK) return s;
+  HIPTRY(c, hipMemcpy(c->canon.pb.Jp, Jp, (size_t)c->canon.cap * 4, hipMemcpyHostToDevice));
+  c->work_valid = false;
+  return AEMH_OK;
+}
+
+aemh_status aemh_download_particles(aemh_ctx* c, float* pos3, float* vel3, float* C9, float* F9,
+                                    float* J) {
+  if (!c) return AEMH_ERR_ARG;
+  if (!c->canon.slab) return fail(c, AEMH_ERR_STATE, "no particles uploaded");
+  hipSetDevice(c->device);
+  aemh_status s = flush_work(c);
+  if (s != AEMH_OK) return s;
+  const uint32_t n = c->canon.cap;
+  PBuf& P = c->canon.pb;
+  if ((s = download_aos(c, pos3, P.x, 3, n)) != AEMH_OK) return s;
+  if ((s = download_aos(c, vel3, P.v, 3, n)) != AEMH_OK) return s;
+  if ((s = download_aos(c, C9, P.C, 9, n)) != AEMH_OK) return s;
+  if ((s = download_aos(c, F9, P.F, 9, n)) != AEMH_OK) return s;
+  if (J) HIPTRY(c, hipMemcpy(J, P.J, (size_t)n * 4, hipMemcpyDeviceToHost));
+  return AEMH_OK;
+}
+
+aemh_status aemh_download_plastic_j(aemh_ctx* c, float* Jp) {
+  if (!c || !Jp) return fail(c, AEMH_ERR_ARG, "null argument");
+  if (!c->canon.slab) return fail(c, AEMH_ERR_STATE, "no particles uploaded");
+  hipSetDevice(c->device);
+  aemh_status s = flush_work(c);
+  if (s != AEMH_OK) return s;
+  HIPTRY(c, hipMemcpy(Jp, c->canon.pb.Jp, (size_t)c->canon.cap * 4, hipMemcpyDeviceToHost));
+  return AEMH_OK;
+}
+
+aemh_status aemh_set_indices(aemh_ctx* c, const uint32_t* idx, uint32_t n_idx) {
+  if (!c) return AEMH_ERR_ARG;
+  if (!c->canon.slab) return fail(c, AEMH_ERR_STATE, "no particles uploaded");
+  if (n_idx && !idx) return fail(c, AEMH_ERR_ARG, "null index list");
+  hipSetDevice(c->device);
+  // Distinct, in-range indices (the reference relies on this by convention,
+  // mpm_solver.rs:1462; a duplicate would make two G2P writers race here).
+  std::vector<uint8_t> seen(c->canon.cap, 0);
+  for (uint32_t i = 0; i < n_idx; i++) {
+    if (idx[i] >= c->canon.cap)
+      return fail(c, AEMH_ERR_ARG, "index %u out of range (capacity %u)", idx[i], c->canon.cap);
+    if (seen[idx[i]]) return fail(c, AEMH_ERR_ARG, "duplicate particle index %u", idx[i]);
+    seen[idx[i]] = 1;
+  }
+  aemh_status s = flush_work(c);
+  if (s != AEMH_OK) return s;
+  const int storage = (int)c->cfg.storage;
+  for (int k = 0; k < 2; k++)
+    if ((s = alloc_store(c, c->work[k], n_idx ? n_idx : 1, storage)) != AEMH_OK) return s;
+  if (c->idx_d) hipFree(c->idx_d);
+  c->idx_d = nullptr;
+  HIPTRY(c, hipMalloc(&c->idx_d, (size_t)(n_idx ? n_idx : 1) * 4));
+  if (n_idx) HIPTRY(c, hipMemcpy(c->idx_d, idx, (size_t)n_idx * 4, hipMemcpyHostToDevice));
+  c->idx_h.assign(idx, idx + n_idx);
+  c->n_work = n_idx;
+  c->cur = 0;
+  launch_gather_work(c->stream, c->canon.pb, c->work[0].pb, c->idx_d, n_idx, storage);
+  CHECK_LAUNCH(c);
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  c->work_valid = true;
+  c->work_dirty = false;
+  c->batch_mode = false;
+  invalidate_graphs(c);
+  return AEMH_OK;
+}
+
+aemh_status aemh_step(aemh_ctx* c, float total_dt) {
+  if (!c) return AEMH_ERR_ARG;
+  hipSetDevice(c->device);
+  return single_scene(c, total_dt, 0);
+}
+
+aemh_status aemh_mpm_step_parallel_with_grid(aemh_ctx* c, const uint32_t* idx, uint32_t n_idx,
+                                             float total_dt) {
+  if (!c) return AEMH_ERR_ARG;
+  if (n_idx == 0) return AEMH_OK;  // mpm_solver.rs:1072-1074
+  hipSetDevice(c->device);
+  if (!c->work_valid || c->idx_h.size() != n_idx ||
+      memcmp(c->idx_h.data(), idx, (size_t)n_idx * 4) != 0) {
+    aemh_status s = aemh_set_indices(c, idx, n_idx);
+    if (s != AEMH_OK) return s;
+  }
+  return single_scene(c, total_dt, 0);
+}
+
+aemh_status aemh_step_velocity_only(aemh_ctx* c, float total_dt) {
+  if (!c) return AEMH_ERR_ARG;
+  hipSetDevice(c->device);
+  return single_scene(c, total_dt, 1);
+}
+
+aemh_status aemh_set_scenes(aemh_ctx* c, const aemh_scene* scenes, uint32_t n) {
+  if (!c) return AEMH_ERR_ARG;
+  if (n == 0 || !scenes) return fail(c, AEMH_ERR_ARG, "need at least one scene");
+  hipSetDevice(c->device);
+  aemh_status s = ensure_work(c);
+  if (s != AEMH_OK) return s;
+  c->scenes_h.assign(scenes, scenes + n);
+  s = set_scene_table(c, c->scenes_h);
+  if (s != AEMH_OK) return s;
+  c->batch_mode = true;
+  return AEMH_OK;
+}
+
+aemh_status aemh_batch_step(aemh_ctx* c, uint32_t n_substeps) {
+  if (!c) return AEMH_ERR_ARG;
+  if (!c->batch_mode) return fail(c, AEMH_ERR_STATE, "aemh_set_scenes first");
+  hipSetDevice(c->device);
+  if (!c->tuned) {
+    aemh_status s = aemh_autotune(c);
+    if (s != AEMH_OK) return s;
+  }
+  return run_substeps(c, n_substeps);
+}
+
+aemh_status aemh_autotune(aemh_ctx* c) {
+  if (!c) return AEMH_ERR_ARG;
+  c->tuned = true;
+  return AEMH_OK;
+}
+
+aemh_status aemh_download_grid(aemh_ctx* c, uint32_t scene, float* mass, float* momentum3,
+                               float* force3, float* vel3) {
+  if (!c) return AEMH_ERR_ARG;
+  if (scene >= c->scd_h.size()) return fail(c, AEMH_ERR_ARG, "no such scene");
+  hipSetDevice(c->device);
+  const SceneDev& S = c->scd_h[scene];
+  const size_t nodes = (size_t)S.n[0] * S.n[1] * S.n[2];
+  float* tmp = nullptr;
+  HIPTRY(c, hipMalloc(&tmp, nodes * 10 * 4));
+  HIPTRY(c, hipMemsetAsync(tmp, 0, nodes * 10 * 4, c->stream));
+  launch_grid_readback(c->stream, c->scd_d, (int)scene, S, c->sort, c->grid,
+                       c->cfg.accum == AEMH_ACCUM_Q32_32, tmp, tmp + nodes, tmp + 4 * nodes,
+                       tmp + 7 * nodes, nullptr, c->shape.upd_wgs);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess && mass) e = hipMemcpy(mass, tmp, nodes * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && momentum3) e = hipMemcpy(momentum3, tmp + nodes, nodes * 12, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && force3) e = hipMemcpy(force3, tmp + 4 * nodes, nodes * 12, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && vel3) e = hipMemcpy(vel3, tmp + 7 * nodes, nodes * 12, hipMemcpyDeviceToHost);
+  hipFree(tmp);
+  if (e != hipSuccess) return fail(c, AEMH_ERR_HIP, "grid readback: %s", hipGetErrorName(e));
+  return AEMH_OK;
+}
+
+aemh_status aemh_download_grid_q32(aemh_ctx* c, uint32_t scene, int64_t* q7) {
+  if (!c || !q7) return fail(c, AEMH_ERR_ARG, "null argument");
+  if (scene >= c->scd_h.size()) return fail(c, AEMH_ERR_ARG, "no such scene");
+  if (c->cfg.accum != AEMH_ACCUM_Q32_32) return fail(c, AEMH_ERR_STATE, "not in Q32.32 mode");
+  hipSetDevice(c->device);
+  const SceneDev& S = c->scd_h[scene];
+  const size_t nodes = (size_t)S.n[0] * S.n[1] * S.n[2];
+  long long* tmp = nullptr;
+  HIPTRY(c, hipMalloc(&tmp, nodes * 7 * 8));
+  HIPTRY(c, hipMemsetAsync(tmp, 0, nodes * 7 * 8, c->stream));
+  launch_grid_readback(c->stream, c->scd_d, (int)scene, S, c->sort, c->grid, 1, nullptr, nullptr,
+                       nullptr, nullptr, tmp, c->shape.upd_wgs);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(q7, tmp, nodes * 7 * 8, hipMemcpyDeviceToHost);
+  hipFree(tmp);
+  if (e != hipSuccess) return fail(c, AEMH_ERR_HIP, "grid readback: %s", hipGetErrorName(e));
+  return AEMH_OK;
+}
+
+aemh_status aemh_conservation_f64(aemh_ctx* c, double* out5) {
+  if (!c || !out5) return fail(c, AEMH_ERR_ARG, "null argument");
+  hipSetDevice(c->device);
+  aemh_status s = ensure_work(c);
+  if (s != AEMH_OK) return s;
+  double* tmp = nullptr;
+  HIPTRY(c, hipMalloc(&tmp, (5 + 5 * 512) * sizeof(double)));
+  launch_conservation(c->stream, c->work[c->cur].pb, c->n_work, tmp);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out5, tmp, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(tmp);
+  if (e != hipSuccess) return fail(c, AEMH_ERR_HIP, "reduction: %s", hipGetErrorName(e));
+  return AEMH_OK;
+}
+
+aemh_status aemh_profile_enable(aemh_ctx* c, int on) {
+  if (!c) return AEMH_ERR_ARG;
+  c->prof = on != 0;
+  return AEMH_OK;
+}
+
+aemh_status aemh_profile_get(aemh_ctx* c, const char* kernel, double* avg_ms, uint64_t* count) {
+  if (!c || !kernel) return AEMH_ERR_ARG;
+  auto it = c->prof_rec.find(kernel);
+  if (it == c->prof_rec.end()) {
+    if (avg_ms) *avg_ms = 0;
+    if (count) *count = 0;
+    return AEMH_OK;
+  }
+  if (avg_ms) *avg_ms = it->second.count ? it->second.total_ms / (double)it->second.count : 0.0;
+  if (count) *count = it->second.count;
+  return AEMH_OK;
+}
+
+aemh_status aemh_profile_reset(aemh_ctx* c) {
+  if (!c) return AEMH_ERR_ARG;
+  c->prof_rec.clear();
+  return AEMH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+// aemh_common.h -- POD structures shared by host code and kernels.
+#pragma once
+#include <stdint.h>
+
+namespace aemh {
+
+constexpr int kModelNeoHookean = 0;
+constexpr int kModelDruckerPrager = 1;
+constexpr int kModelVonMises = 2;
+constexpr int kModelNewtonianFluid = 3;
+constexpr int kModelCorotated = 4;
+constexpr int kModelFixedCorotated = 5;
+constexpr int kModelRigid = 6;
+constexpr int kModelSnow = 7;
+
+// Grid is stored block-major: 4x4x4 nodes per block, 64 consecutive nodes
+// (x fastest inside a block), blocks ordered z-y-x over each scene's padded
+// grid and scenes concatenated.  Only blocks touched by particles in the
+// current substep are cleared, updated and read.
+constexpr int kBlockNodes = 64;
+// P2G stages the 6x6x6 node halo of a 4x4x4 block of base cells in LDS.
+constexpr int kTileSide = 6;
+constexpr int kTileNodes = 216;
+
+// One scene as seen by the kernels (derived from aemh_scene + aemh_config).
+struct SceneDev {
+  uint32_t first, count;  // particle range in sorted (working) order
+  int32_t n[3];           // grid nodes
+  int32_t nb[3];          // blocks per axis = ceil(n/4)
+  uint32_t block_off;     // first global block of this scene
+  float dx, inv_dx;
+  float o[3];             // moving-window origin
+  float dt;               // substep dt
+  float g[3];             // gravity
+  float damping;
+  int32_t bc_margin;
+  float bc_rest;
+  uint32_t vel_only;      // velocity-only tier semantics
+  uint32_t window_clamp;  // 1: origin-relative x/z clamp
+  float gmax_x, gmax_z;   // (n-1) dx
+  float four_inv_dx2;     // 4 inv_dx inv_dx, evaluated left to right
+  uint32_t pad;
+};
+
+// Material as the kernels need it.  Derived constants are computed on the
+// host with the reference's f32 operation order (constitutive.rs:55-82).
+struct MatDev {
+  float mu, lambda, bulk, density;
+  float yield_stress, dp_alpha, dp_k, pad0;
+  float theta_c, theta_s, xi, pad1;
+  int32_t model;
+  int32_t pad2[3];
+};
+
+// Particle state in HBM, structure of arrays.  C and F may be stored in
+// 16-bit floats (F as F - I) in the mixed-precision mode.
+struct PBuf {
+  float* x[3];
+  float* v[3];
+  float* C[9];
+  float* F[9];
+  uint16_t* Ch[9];  // 16-bit storage of C (mixed precision)
+  uint16_t* Fh[9];  // 16-bit storage of F - I (mixed precision)
+  float* J;
+  float* Jp;
+  float* mass;
+  uint16_t* mat;
+  uint32_t* id;     // position in the active index list
+};
+
+// Grid accumulators and the updated velocity read by G2P.
+struct GridDev {
+  float* accf;      // [7][nodes] m, p0..2, f0..2 (FP32 mode)
+  long long* accq;  // [7][nodes] Q32.32 raw sums (deterministic mode)
+  float4* gv;       // [nodes] (v.xyz, m) after the grid update
+  uint32_t nodes;
+};
+
+// Per-substep bookkeeping buffers.
+struct SortDev {
+  uint32_t* key;        // [N] global base-block of each working slot
+  uint32_t* rank;       // [N] rank within its block
+  uint32_t* perm;       // [N] sorted position -> working slot
+  uint32_t* cnt;        // [NB] particles per base block (zeroed by the scan)
+  uint32_t* off;        // [NB] tile-local exclusive prefix
+  uint32_t* tsum;       // [tiles] tile sums, scanned in place
+  uint32_t* flag;       // [NB] node block active this substep
+  uint32_t* active;     // [NB] list of active node blocks
+  uint32_t* counters;   // [8] 0: active blocks, 1: overflow flag
+  unsigned long long* qmax;  // max |raw| of a Q32.32 contribution
+};
+
+constexpr int kScanTile = 1024;
+
+}  // namespace aemh

@@ -1,0 +1,238 @@
+// aemh_dev.h -- device-side types and per-particle math of the MLS-MPM step.
+//
+// Included by every HIP translation unit.  The same source is compiled twice:
+// k_fast.hip with -ffp-contract=fast (FP32 mode) and k_det.hip with
+// -ffp-contract=off (deterministic Q32.32 mode, whose per-particle
+// arithmetic must reproduce the reference's rustc evaluation order exactly;
+// rustc never fuses multiply-adds).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "aemh_common.h"
+#include "aemh_half.h"
+
+namespace aemh {
+
+// ---------------------------------------------------------------------------
+// Quadratic B-spline, ae_particle/src/mpm_solver.rs:442-464.  NaN input gives
+// 0 (both comparisons fail), as in the reference.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float b_spline(float x) {
+  float ax = fabsf(x);
+  if (ax < 0.5f) return 0.75f - ax * ax;
+  if (ax < 1.5f) {
+    float t = 1.5f - ax;
+    return 0.5f * t * t;
+  }
+  return 0.0f;
+}
+__device__ __forceinline__ float b_spline_deriv(float x) {
+  float ax = fabsf(x);
+  if (ax < 0.5f) return -2.0f * x;
+  if (ax < 1.5f) {
+    float sg = signbit(x) ? -1.0f : 1.0f;  // f32::signum (x is never NaN here)
+    return -(1.5f - ax) * sg;
+  }
+  return 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Stencil base: base = floor((x - o) inv_dx - 0.5), fx = t - base, base
+// clamped at 0 (mpm_solver.rs:853-861).  Written with _rn intrinsics so the
+// sort key (computed in one translation unit) and P2G/G2P (another, maybe
+// contracted) always agree on the base cell, and match the reference bits.
+// The int conversion is clamped to +-2^30: beyond that the particle lies so
+// far outside any grid that it touches no node either way.
+// ---------------------------------------------------------------------------
+struct Stencil {
+  int g[3];     // clamped base (>= 0)
+  float fx[3];  // fractional position, taken with the unclamped base
+};
+__device__ __forceinline__ void stencil_axis(float x, float o, float inv_dx, int& g,
+                                             float& fx) {
+  float t = __fsub_rn(__fmul_rn(__fsub_rn(x, o), inv_dx), 0.5f);
+  float fl = floorf(t);
+  fx = __fsub_rn(t, fl);
+  float cl = fminf(fmaxf(fl, -1073741824.0f), 1073741824.0f);
+  int gi = (int)cl;
+  g = gi > 0 ? gi : 0;
+}
+__device__ __forceinline__ Stencil make_stencil(const SceneDev& s, float x, float y, float z) {
+  Stencil st;
+  stencil_axis(x, s.o[0], s.inv_dx, st.g[0], st.fx[0]);
+  stencil_axis(y, s.o[1], s.inv_dx, st.g[1], st.fx[1]);
+  stencil_axis(z, s.o[2], s.inv_dx, st.g[2], st.fx[2]);
+  return st;
+}
+// Global block id of a clamped base cell (sort key); bases past the grid are
+// clamped into its last block -- such particles touch no node of that axis.
+__device__ __forceinline__ uint32_t block_key(const SceneDev& s, const Stencil& st) {
+  int bx = min(st.g[0], s.n[0] - 1) >> 2;
+  int by = min(st.g[1], s.n[1] - 1) >> 2;
+  int bz = min(st.g[2], s.n[2] - 1) >> 2;
+  return s.block_off + (uint32_t)((bz * s.nb[1] + by) * s.nb[0] + bx);
+}
+// Node address in the block-major grid (4^3 nodes per block, x fastest).
+__device__ __forceinline__ uint32_t node_addr(const SceneDev& s, int gx, int gy, int gz) {
+  uint32_t b = s.block_off + (uint32_t)(((gz >> 2) * s.nb[1] + (gy >> 2)) * s.nb[0] + (gx >> 2));
+  return (b << 6) | (uint32_t)(((gz & 3) << 4) | ((gy & 3) << 2) | (gx & 3));
+}
+
+// ---------------------------------------------------------------------------
+// Portable ln / exp of the deterministic mode: double-precision evaluation of
+// ln m = 2 atanh((m-1)/(m+1)) and a Taylor exp with 2^k scaling, rounded once
+// to float.  Same operation sequence as the oracle's det_logf/det_expf, so the
+// host and the GPU produce identical bits without fused operations.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double det_log_d(double x) {
+  if (x != x) return x;
+  if (x <= 0.0) return x == 0.0 ? -__builtin_huge_val() : __builtin_nan("");
+  if (x == __builtin_huge_val()) return x;
+  uint64_t b = __double_as_longlong(x);
+  int e = (int)((b >> 52) & 0x7ff) - 1023;
+  double m = __longlong_as_double((long long)((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL));
+  if (m > 1.4142135623730951) {
+    m = __dmul_rn(m, 0.5);
+    e += 1;
+  }
+  double s = __ddiv_rn(__dsub_rn(m, 1.0), __dadd_rn(m, 1.0));
+  double s2 = __dmul_rn(s, s);
+  double p = 1.0 / 23.0;
+  p = __dadd_rn(__dmul_rn(p, s2), 1.0 / 21.0);
+  p = __dadd_rn(__dmul_rn(p, s2), 1.0 / 19.0);
+  p = __dadd_rn(__dmul_rn(p, s2), 1.0 / 17.0);
+  p = __dadd_rn(__dmul_rn(p, s2), 1.0 / 15.0);
+  p = __dadd_rn(__dmul_rn(p, s2), 1.0 / 13.0);
+  p = __dadd_rn(__dmul_rn(p, s2), 1.0 / 11.0);
+  p = __dadd_rn(__dmul_rn(p, s2), 1.0 / 9.0);
+  p = __dadd_rn(__dmul_rn(p, s2), 1.0 / 7.0);
+  p = __dadd_rn(__dmul_rn(p, s2), 1.0 / 5.0);
+  p = __dadd_rn(__dmul_rn(p, s2), 1.0 / 3.0);
+  p = __dadd_rn(__dmul_rn(p, s2), 1.0);
+  double r = __dmul_rn(__dmul_rn(2.0, s), p);
+  double ed = (double)e;
+  const double ln2hi = 6.93147180369123816490e-01, ln2lo = 1.90821492927058770002e-10;
+  return __dadd_rn(__dmul_rn(ed, ln2hi), __dadd_rn(__dmul_rn(ed, ln2lo), r));
+}
+__device__ __forceinline__ double det_exp_d(double x) {
+  if (x != x) return x;
+  if (x > 709.0) return __builtin_huge_val();
+  if (x < -745.0) return 0.0;
+  const double ln2hi = 6.93147180369123816490e-01, ln2lo = 1.90821492927058770002e-10;
+  const double invln2 = 1.44269504088896338700e+00;
+  double kf = floor(__dadd_rn(__dmul_rn(x, invln2), 0.5));
+  double r = __dsub_rn(__dsub_rn(x, __dmul_rn(kf, ln2hi)), __dmul_rn(kf, ln2lo));
+  double p = 1.0 / 87178291200.0;
+  p = __dadd_rn(__dmul_rn(p, r), 1.0 / 6227020800.0);
+  p = __dadd_rn(__dmul_rn(p, r), 1.0 / 479001600.0);
+  p = __dadd_rn(__dmul_rn(p, r), 1.0 / 39916800.0);
+  p = __dadd_rn(__dmul_rn(p, r), 1.0 / 3628800.0);
+  p = __dadd_rn(__dmul_rn(p, r), 1.0 / 362880.0);
+  p = __dadd_rn(__dmul_rn(p, r), 1.0 / 40320.0);
+  p = __dadd_rn(__dmul_rn(p, r), 1.0 / 5040.0);
+  p = __dadd_rn(__dmul_rn(p, r), 1.0 / 720.0);
+  p = __dadd_rn(__dmul_rn(p, r), 1.0 / 120.0);
+  p = __dadd_rn(__dmul_rn(p, r), 1.0 / 24.0);
+  p = __dadd_rn(__dmul_rn(p, r), 1.0 / 6.0);
+  p = __dadd_rn(__dmul_rn(p, r), 0.5);
+  p = __dadd_rn(__dmul_rn(p, r), 1.0);
+  p = __dadd_rn(__dmul_rn(p, r), 1.0);
+  int k = (int)kf;
+  int k1 = k / 2, k2 = k - k1;
+  double s1 = __longlong_as_double((long long)((uint64_t)(k1 + 1023) << 52));
+  double s2 = __longlong_as_double((long long)((uint64_t)(k2 + 1023) << 52));
+  return __dmul_rn(__dmul_rn(p, s1), s2);
+}
+__device__ __forceinline__ float det_logf(float x) { return (float)det_log_d((double)x); }
+__device__ __forceinline__ float det_expf(float x) {
+  if (x > 89.0f) return __builtin_huge_valf();
+  if (x < -104.0f) return 0.0f;
+  return (float)det_exp_d((double)x);
+}
+
+// ---------------------------------------------------------------------------
+// FixedPoint Q32.32 (ae_physics/src/fixed_point.rs:22-38): from_f32 truncates
+// (f64(v) 2^32) toward zero with saturation, NaN -> 0; to_f32 = raw/2^32.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ long long fp_from_f32(float v) {
+  double d = (double)v * 4294967296.0;  // exact
+  if (d != d) return 0;
+  if (d >= 9223372036854775807.0) return 0x7fffffffffffffffLL;
+  if (d <= -9223372036854775808.0) return (long long)0x8000000000000000ULL;
+  return (long long)d;
+}
+__device__ __forceinline__ float fp_to_f32(long long raw) {
+  return __fdiv_rn((float)raw, 4294967296.0f);
+}
+
+// Row-major 3x3 determinant in the order of mpm_solver.rs:1038-1040.
+template <typename T>
+__device__ __forceinline__ T det3(const T* f) {
+  return f[0] * (f[4] * f[8] - f[5] * f[7]) - f[1] * (f[3] * f[8] - f[5] * f[6]) +
+         f[2] * (f[3] * f[7] - f[4] * f[6]);
+}
+
+}  // namespace aemh
+
+namespace aemh {
+
+// Scene of a sorted particle position: scenes hold contiguous, ascending
+// particle ranges, so the last scene with first <= i owns i.
+__device__ __forceinline__ int scene_of(const SceneDev* __restrict__ sc, int ns, uint32_t i) {
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (sc[mid].first <= i) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+// Scene of a global block id (block ranges are ascending by scene).
+__device__ __forceinline__ int scene_of_block(const SceneDev* __restrict__ sc, int ns, uint32_t b) {
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (sc[mid].block_off <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Wave-wide inclusive prefix sum (wave64).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+}  // namespace aemh
+
+namespace aemh {
+
+// C and F accessors honouring the storage mode (0 FP32, 1 BF16, 2 FP16;
+// F is stored as F - I in 16-bit modes).
+__device__ __forceinline__ void put_c(PBuf& P, uint32_t j, int k, float c, int storage) {
+  if (storage == 0) P.C[k][j] = c;
+  else P.Ch[k][j] = storage == 1 ? f32_to_bf16(c) : f32_to_f16(c);
+}
+__device__ __forceinline__ void put_f(PBuf& P, uint32_t j, int k, float f, int storage) {
+  if (storage == 0) P.F[k][j] = f;
+  else {
+    float d = f - ((k % 4 == 0) ? 1.0f : 0.0f);
+    P.Fh[k][j] = storage == 1 ? f32_to_bf16(d) : f32_to_f16(d);
+  }
+}
+__device__ __forceinline__ float get_c(const PBuf& P, uint32_t j, int k, int storage) {
+  if (storage == 0) return P.C[k][j];
+  return storage == 1 ? bf16_to_f32(P.Ch[k][j]) : f16_to_f32(P.Ch[k][j]);
+}
+__device__ __forceinline__ float get_f(const PBuf& P, uint32_t j, int k, int storage) {
+  if (storage == 0) return P.F[k][j];
+  float d = storage == 1 ? bf16_to_f32(P.Fh[k][j]) : f16_to_f32(P.Fh[k][j]);
+  return d + ((k % 4 == 0) ? 1.0f : 0.0f);
+}
+
+}  // namespace aemh

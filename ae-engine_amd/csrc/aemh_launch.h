@@ -1,0 +1,53 @@
+// aemh_launch.h -- host-callable launchers for the kernels of each
+// translation unit.  All launches go to the caller's stream and are
+// asynchronous; grid sizes depend only on sizes the host knows, so a whole
+// substep can be captured in a hipGraph.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "aemh_common.h"
+
+namespace aemh {
+
+struct LaunchShape {
+  int p2g_wg = 256;      // particles per P2G workgroup (one per thread)
+  int g2p_wg = 256;      // G2P workgroup size
+  int upd_wgs = 2048;    // persistent grid-update workgroups (4 blocks each)
+};
+
+// ---- k_common.hip --------------------------------------------------------
+void launch_step_begin(hipStream_t st, SortDev s);
+void launch_count(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, uint32_t n, SortDev s);
+void launch_scan(hipStream_t st, SortDev s, uint32_t nb);
+void launch_scatter(hipStream_t st, uint32_t n, SortDev s);
+void launch_clear(hipStream_t st, SortDev s, GridDev g, int q32, int upd_wgs);
+// canonical (capacity) <-> working set
+void launch_gather_work(hipStream_t st, PBuf canon, PBuf work, const uint32_t* idx, uint32_t n,
+                        int storage);
+void launch_scatter_work(hipStream_t st, PBuf work, PBuf canon, const uint32_t* idx, uint32_t n,
+                         int storage);
+void launch_fill(hipStream_t st, float* p, float v, uint32_t n);
+// Dense x-fastest readback of one scene's grid (active blocks only).
+void launch_grid_readback(hipStream_t st, const SceneDev* sc_d, int scene, SceneDev sc,
+                          SortDev s, GridDev g, int q32, float* m, float* p3, float* f3,
+                          float* v3, long long* q7, int upd_wgs);
+// FP64 sums over the working set: mass, momentum xyz, kinetic energy.
+void launch_conservation(hipStream_t st, PBuf cur, uint32_t n, double* out5);
+
+// ---- k_fast.hip (FP32 mode) and k_det.hip (deterministic mode) -----------
+#define AEMH_DECLARE_MODE(NS)                                                               \
+  namespace NS {                                                                            \
+  void launch_p2g(hipStream_t st, const LaunchShape& ls, PBuf cur, const SceneDev* sc,      \
+                  int ns, const MatDev* mats, int nmat, uint32_t n, SortDev s, GridDev g,   \
+                  int storage);                                                             \
+  void launch_grid_update(hipStream_t st, const LaunchShape& ls, const SceneDev* sc,        \
+                          int ns, SortDev s, GridDev g);                                    \
+  void launch_g2p(hipStream_t st, const LaunchShape& ls, PBuf cur, PBuf nxt,                \
+                  const SceneDev* sc, int ns, const MatDev* mats, int nmat, uint32_t n,     \
+                  SortDev s, GridDev g, int storage);                                       \
+  }
+AEMH_DECLARE_MODE(fast)
+AEMH_DECLARE_MODE(det)
+#undef AEMH_DECLARE_MODE
+
+}  // namespace aemh

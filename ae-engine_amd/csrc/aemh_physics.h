@@ -1,0 +1,327 @@
+// aemh_physics.h -- per-particle constitutive update on the GPU.
+//
+// Included by k_fast.hip (AEMH_DET=0) and k_det.hip (AEMH_DET=1).  In the
+// deterministic build the expressions keep the evaluation order of the
+// reference (ae_particle/src/constitutive.rs) and transcendental functions
+// come from the portable det_logf/det_expf, so results match a CPU build of
+// the same arithmetic bit for bit.
+#pragma once
+#include "aemh_dev.h"
+
+#ifndef AEMH_DET
+#error "AEMH_DET must be defined to 0 or 1"
+#endif
+
+namespace aemh {
+namespace {
+
+#if AEMH_DET
+__device__ __forceinline__ float m_ln(float x) { return det_logf(x); }
+__device__ __forceinline__ float m_exp(float x) { return det_expf(x); }
+#else
+__device__ __forceinline__ float m_ln(float x) { return logf(x); }
+__device__ __forceinline__ float m_exp(float x) { return expf(x); }
+#endif
+
+template <typename T>
+__device__ __forceinline__ void identity3(T* m) {
+#pragma unroll
+  for (int i = 0; i < 9; i++) m[i] = (i % 4 == 0) ? T(1) : T(0);
+}
+// constitutive.rs:264-276
+template <typename T>
+__device__ __forceinline__ void mat_mul(const T* a, const T* b, T* o) {
+  T t[9];
+  t[0] = a[0] * b[0] + a[1] * b[3] + a[2] * b[6];
+  t[1] = a[0] * b[1] + a[1] * b[4] + a[2] * b[7];
+  t[2] = a[0] * b[2] + a[1] * b[5] + a[2] * b[8];
+  t[3] = a[3] * b[0] + a[4] * b[3] + a[5] * b[6];
+  t[4] = a[3] * b[1] + a[4] * b[4] + a[5] * b[7];
+  t[5] = a[3] * b[2] + a[4] * b[5] + a[5] * b[8];
+  t[6] = a[6] * b[0] + a[7] * b[3] + a[8] * b[6];
+  t[7] = a[6] * b[1] + a[7] * b[4] + a[8] * b[7];
+  t[8] = a[6] * b[2] + a[7] * b[5] + a[8] * b[8];
+#pragma unroll
+  for (int i = 0; i < 9; i++) o[i] = t[i];
+}
+template <typename T>
+__device__ __forceinline__ void transpose3(const T* m, T* o) {
+  T t[9] = {m[0], m[3], m[6], m[1], m[4], m[7], m[2], m[5], m[8]};
+#pragma unroll
+  for (int i = 0; i < 9; i++) o[i] = t[i];
+}
+
+// `inv_transpose` of constitutive.rs:306-324 (returns adjugate/det as written).
+__device__ __forceinline__ void inv_transpose(const float* m, float* o) {
+  float det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+              m[2] * (m[3] * m[7] - m[4] * m[6]);
+  if (fabsf(det) < 1e-15f) {
+    identity3(o);
+    return;
+  }
+  float inv_det = 1.0f / det;
+  o[0] = (m[4] * m[8] - m[5] * m[7]) * inv_det;
+  o[1] = -(m[1] * m[8] - m[2] * m[7]) * inv_det;
+  o[2] = (m[1] * m[5] - m[2] * m[4]) * inv_det;
+  o[3] = -(m[3] * m[8] - m[5] * m[6]) * inv_det;
+  o[4] = (m[0] * m[8] - m[2] * m[6]) * inv_det;
+  o[5] = -(m[0] * m[5] - m[2] * m[3]) * inv_det;
+  o[6] = (m[3] * m[7] - m[4] * m[6]) * inv_det;
+  o[7] = -(m[0] * m[7] - m[1] * m[6]) * inv_det;
+  o[8] = (m[0] * m[4] - m[1] * m[3]) * inv_det;
+}
+
+// polar_decomp, constitutive.rs:284-304.
+__device__ __noinline__ void polar_decomp(const float* F, float* r, float* s) {
+#pragma unroll
+  for (int i = 0; i < 9; i++) r[i] = F[i];
+  for (int it = 0; it < 8; it++) {
+    float rit[9], nr[9];
+    inv_transpose(r, rit);
+    float diff = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      nr[i] = 0.5f * (r[i] + rit[i]);
+      diff += fabsf(nr[i] - r[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 9; i++) r[i] = nr[i];
+    if (diff < 1e-10f) break;
+  }
+  float rt[9];
+  transpose3(r, rt);
+  mat_mul(rt, F, s);
+}
+
+// neo_hookean, constitutive.rs:375-393, with mu/lambda as arguments.
+__device__ __forceinline__ void neo_hookean(const float* F, float J, float mu, float lambda,
+                                            float* st) {
+  float ft[9], b[9];
+  transpose3(F, ft);
+  mat_mul(F, ft, b);
+  float p_term = (J > 0.0f) ? lambda * m_ln(J) / J : 0.0f;
+  float scale = mu / fmaxf(J, 1e-10f);
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    float dev = (i % 4 == 0) ? b[i] - 1.0f + p_term / scale : b[i];
+    st[i] = scale * dev;
+  }
+}
+
+// Drucker-Prager / von Mises, constitutive.rs:443-601 (stress only; the
+// corrected F is discarded by P2G, mpm_solver.rs:843-844).
+__device__ __noinline__ void plastic_stress(const MatDev& M, const float* F, float* st) {
+  float r[9], s[9];
+  polar_decomp(F, r, s);
+  const float mu = M.mu, lambda = M.lambda;
+  float eps[9] = {s[0] - 1.0f, s[1], s[2], s[3], s[4] - 1.0f, s[5], s[6], s[7], s[8] - 1.0f};
+  float trace_e = eps[0] + eps[4] + eps[8];
+  float tr3 = trace_e / 3.0f;
+  eps[0] -= tr3;
+  eps[4] -= tr3;
+  eps[8] -= tr3;
+  float sd[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) sd[i] = 2.0f * mu * eps[i];
+  float s_norm = sqrtf(0.5f * (sd[0] * sd[0] + sd[4] * sd[4] + sd[8] * sd[8] +
+                               2.0f * (sd[1] * sd[1] + sd[2] * sd[2] + sd[3] * sd[3] +
+                                       sd[5] * sd[5] + sd[6] * sd[6] + sd[7] * sd[7])));
+  float pressure = -lambda * trace_e;
+  if (M.model == kModelDruckerPrager) {
+    float yield_val = s_norm + M.dp_alpha * pressure - M.dp_k;
+    if (yield_val > 0.0f) {
+      float scale = fmaxf(M.dp_k - M.dp_alpha * pressure, 0.0f) / fmaxf(s_norm, 1e-10f);
+#pragma unroll
+      for (int i = 0; i < 9; i++) sd[i] *= scale;
+    }
+  } else {
+    float yield_threshold = M.yield_stress / sqrtf(3.0f);
+    if (s_norm > yield_threshold) {
+      float scale = yield_threshold / s_norm;
+#pragma unroll
+      for (int i = 0; i < 9; i++) sd[i] *= scale;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 9; i++) st[i] = sd[i];
+  st[0] -= pressure;
+  st[4] -= pressure;
+  st[8] -= pressure;
+}
+
+__device__ __noinline__ void corotated_stress(const MatDev& M, const float* F, float J, float* st) {
+  float r[9], s[9];
+  polar_decomp(F, r, s);
+  if (M.model == kModelCorotated) {  // constitutive.rs:395-415
+#pragma unroll
+    for (int i = 0; i < 9; i++) st[i] = 2.0f * M.mu * (F[i] - r[i]);
+    st[0] += M.lambda * (J - 1.0f);
+    st[4] += M.lambda * (J - 1.0f);
+    st[8] += M.lambda * (J - 1.0f);
+  } else {  // fixed_corotated, constitutive.rs:417-441
+    float rt[9], diff[9], drt[9];
+    transpose3(r, rt);
+#pragma unroll
+    for (int i = 0; i < 9; i++) diff[i] = F[i] - r[i];
+    mat_mul(diff, rt, drt);
+#pragma unroll
+    for (int i = 0; i < 9; i++) st[i] = 2.0f * M.mu * drt[i];
+    st[0] += M.lambda * (J - 1.0f) * J;
+    st[4] += M.lambda * (J - 1.0f) * J;
+    st[8] += M.lambda * (J - 1.0f) * J;
+  }
+}
+
+// compute_stress dispatch, constitutive.rs:234-249, plus the snow model
+// (Stomakhin 2013 hardening h = exp(xi (1 - J_p)) clamped to [0.1, 5] on
+// mu and lambda of the Neo-Hookean stress; no reference counterpart).
+__device__ __forceinline__ void compute_stress(const MatDev& M, const float* F, float J, float Jp,
+                                               float* st) {
+  switch (M.model) {
+    case kModelRigid:
+      identity3(st);
+      break;
+    case kModelNewtonianFluid: {
+      float sd = M.bulk * (J - 1.0f);
+#pragma unroll
+      for (int i = 0; i < 9; i++) st[i] = 0.0f;
+      st[0] = sd;
+      st[4] = sd;
+      st[8] = sd;
+      break;
+    }
+    case kModelCorotated:
+    case kModelFixedCorotated:
+      corotated_stress(M, F, J, st);
+      break;
+    case kModelDruckerPrager:
+    case kModelVonMises:
+      plastic_stress(M, F, st);
+      break;
+    case kModelSnow: {
+      float h = m_exp(M.xi * (1.0f - Jp));
+      h = fminf(fmaxf(h, 0.1f), 5.0f);
+      neo_hookean(F, J, M.mu * h, M.lambda * h, st);
+      break;
+    }
+    default:
+      neo_hookean(F, J, M.mu, M.lambda, st);
+      break;
+  }
+}
+
+// One-sided Jacobi SVD (Hestenes), 6 sweeps over (0,1),(0,2),(1,2); U and V
+// rotations, a reflection carried by the smallest singular value.  Same
+// operation sequence as the oracle's svd3_jacobi.
+template <typename T>
+__device__ __noinline__ void svd3_jacobi(const T* F, T* U, T* sig, T* V) {
+  T a[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) a[i] = F[i];
+  identity3(V);
+  for (int sweep = 0; sweep < 6; sweep++) {
+#pragma unroll
+    for (int pq = 0; pq < 3; pq++) {
+      const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
+      T alpha = a[p] * a[p] + a[3 + p] * a[3 + p] + a[6 + p] * a[6 + p];
+      T beta = a[q] * a[q] + a[3 + q] * a[3 + q] + a[6 + q] * a[6 + q];
+      T gamma = a[p] * a[q] + a[3 + p] * a[3 + q] + a[6 + p] * a[6 + q];
+      if (gamma == T(0)) continue;
+      T zeta = (beta - alpha) / (T(2.0) * gamma);
+      T t = T(1.0) / (fabs(zeta) + sqrt(T(1.0) + zeta * zeta));
+      if (zeta < T(0)) t = -t;
+      T c = T(1.0) / sqrt(T(1.0) + t * t);
+      T s = c * t;
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        T ap = a[3 * r + p], aq = a[3 * r + q];
+        a[3 * r + p] = c * ap - s * aq;
+        a[3 * r + q] = s * ap + c * aq;
+        T vp = V[3 * r + p], vq = V[3 * r + q];
+        V[3 * r + p] = c * vp - s * vq;
+        V[3 * r + q] = s * vp + c * vq;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    T n2 = a[c] * a[c] + a[3 + c] * a[3 + c] + a[6 + c] * a[6 + c];
+    sig[c] = sqrt(n2);
+  }
+  int zero_col = -1;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (sig[c] > T(1e-30)) {
+      T inv = T(1.0) / sig[c];
+      for (int r = 0; r < 3; r++) U[3 * r + c] = a[3 * r + c] * inv;
+    } else {
+      zero_col = c;
+      for (int r = 0; r < 3; r++) U[3 * r + c] = T(0);
+    }
+  }
+  if (zero_col >= 0) {
+    int c1 = (zero_col + 1) % 3, c2 = (zero_col + 2) % 3;
+    T x = U[3 * 1 + c1] * U[3 * 2 + c2] - U[3 * 2 + c1] * U[3 * 1 + c2];
+    T y = U[3 * 2 + c1] * U[3 * 0 + c2] - U[3 * 0 + c1] * U[3 * 2 + c2];
+    T z = U[3 * 0 + c1] * U[3 * 1 + c2] - U[3 * 1 + c1] * U[3 * 0 + c2];
+    T n = sqrt(x * x + y * y + z * z);
+    if (n > T(1e-30)) {
+      U[zero_col] = x / n;
+      U[3 + zero_col] = y / n;
+      U[6 + zero_col] = z / n;
+    } else {
+      identity3(U);
+    }
+  }
+  T du = det3(U);
+  if (du < T(0)) {
+    int k = 0;
+    if (sig[1] < sig[k]) k = 1;
+    if (sig[2] < sig[k]) k = 2;
+    sig[k] = -sig[k];
+    for (int r = 0; r < 3; r++) U[3 * r + k] = -U[3 * r + k];
+  }
+}
+
+// FP32 SVD with the FP64 fallback for cond(F) > 1e4.
+__device__ __forceinline__ void svd3(const float* F, float* U, float* sig, float* V) {
+  svd3_jacobi<float>(F, U, sig, V);
+  float smax = fmaxf(fabsf(sig[0]), fmaxf(fabsf(sig[1]), fabsf(sig[2])));
+  float smin = fminf(fabsf(sig[0]), fminf(fabsf(sig[1]), fabsf(sig[2])));
+  if (!(smin * 1e4f >= smax)) {
+    double Fd[9], Ud[9], sd[3], Vd[9];
+    for (int i = 0; i < 9; i++) Fd[i] = F[i];
+    svd3_jacobi<double>(Fd, Ud, sd, Vd);
+    for (int i = 0; i < 9; i++) {
+      U[i] = (float)Ud[i];
+      V[i] = (float)Vd[i];
+    }
+    for (int i = 0; i < 3; i++) sig[i] = (float)sd[i];
+  }
+}
+
+// Snow plasticity after the F update (Stomakhin et al. 2013): clamp the
+// singular values to [1 - theta_c, 1 + theta_s], J_p *= sigma / sigma_hat,
+// F = U diag(sigma_hat) V^T.  No reference counterpart in the FP32 solver.
+__device__ __noinline__ void snow_clamp(const MatDev& M, float* F, float* Jp) {
+  float U[9], s[3], V[9];
+  svd3(F, U, s, V);
+  const float lo = 1.0f - M.theta_c, hi = 1.0f + M.theta_s;
+  float sh[3];
+  float jp = *Jp;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    sh[d] = fminf(fmaxf(s[d], lo), hi);
+    jp = jp * s[d] / sh[d];
+  }
+  *Jp = jp;
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      F[3 * r + c] = U[3 * r + 0] * sh[0] * V[3 * c + 0] + U[3 * r + 1] * sh[1] * V[3 * c + 1] +
+                     U[3 * r + 2] * sh[2] * V[3 * c + 2];
+}
+
+}  // namespace
+}  // namespace aemh

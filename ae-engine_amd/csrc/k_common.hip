@@ -1,0 +1,304 @@
+// k_common.hip -- kernels shared by both arithmetic modes: counting sort of
+// particles by base block, active-block bookkeeping, grid clears, layout
+// conversions between the reference's AoS arrays and HBM structure of
+// arrays, parity readback and FP64 reductions.
+#include "aemh_dev.h"
+#include "aemh_launch.h"
+
+namespace aemh {
+namespace {
+
+__global__ void k_step_begin(SortDev s) {
+  if (threadIdx.x < 8) s.counters[threadIdx.x] = 0;
+}
+
+// Key + rank: every working slot computes the global block of its clamped
+// base cell; runs of equal keys inside a wave take one atomic on the block
+// counter, so nearly sorted input costs about one atomic per wave.  Each run
+// head also activates the 2x2x2 node blocks its 6^3 halo can touch.
+__global__ __launch_bounds__(256) void k_count(PBuf cur, const SceneDev* __restrict__ sc, int ns,
+                                               uint32_t n, SortDev s) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const bool valid = i < n;
+  if (__ballot(valid) == 0) return;
+  const int lane = __lane_id();
+  uint32_t key = 0xffffffffu;
+  int bc[3] = {0, 0, 0};
+  int sid = 0;
+  if (valid) {
+    sid = scene_of(sc, ns, i);
+    const SceneDev& S = sc[sid];
+    Stencil st = make_stencil(S, cur.x[0][i], cur.x[1][i], cur.x[2][i]);
+    for (int a = 0; a < 3; a++) bc[a] = min(st.g[a], S.n[a] - 1) >> 2;
+    key = S.block_off + (uint32_t)((bc[2] * S.nb[1] + bc[1]) * S.nb[0] + bc[0]);
+    s.key[i] = key;
+  }
+  const uint32_t prev = __shfl_up(key, 1, 64);
+  const bool head = valid && (lane == 0 || prev != key);
+  const unsigned long long heads = __ballot(head);
+  const int nvalid = __popcll(__ballot(valid));
+  const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  uint32_t old = 0;
+  if (head) {
+    const unsigned long long above = heads & ~le;
+    const int next = above ? (__ffsll((long long)above) - 1) : nvalid;
+    old = atomicAdd(&s.cnt[key], (uint32_t)(next - lane));
+    const SceneDev& S = sc[sid];
+    for (int dz = 0; dz < 2; dz++)
+      for (int dy = 0; dy < 2; dy++)
+        for (int dx = 0; dx < 2; dx++) {
+          int bx = bc[0] + dx, by = bc[1] + dy, bz = bc[2] + dz;
+          if (bx >= S.nb[0] || by >= S.nb[1] || bz >= S.nb[2]) continue;
+          uint32_t b = S.block_off + (uint32_t)((bz * S.nb[1] + by) * S.nb[0] + bx);
+          if (__atomic_load_n(&s.flag[b], __ATOMIC_RELAXED) == 0u &&
+              atomicExch(&s.flag[b], 1u) == 0u) {
+            uint32_t slot = atomicAdd(&s.counters[0], 1u);
+            s.active[slot] = b;
+          }
+        }
+  }
+  const unsigned long long mine = valid ? (heads & le) : 1ull;
+  const int h = 63 - __clzll((long long)mine);
+  const uint32_t base = __shfl(old, h, 64);
+  if (valid) s.rank[i] = base + (uint32_t)(lane - h);
+}
+
+// Exclusive scan of the per-block counts, tile of 1024 per workgroup; zeroes
+// the counts for the next substep.
+__global__ __launch_bounds__(256) void k_scan_tiles(SortDev s, uint32_t nb) {
+  __shared__ uint32_t wtot[4];
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 4u;
+  uint32_t v[4];
+  for (int k = 0; k < 4; k++) {
+    v[k] = (base + k < nb) ? s.cnt[base + k] : 0u;
+    if (base + k < nb) s.cnt[base + k] = 0u;
+  }
+  const uint32_t sum = v[0] + v[1] + v[2] + v[3];
+  const uint32_t incl = wave_incl_scan(sum);
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  if (lane == 63) wtot[wave] = incl;
+  __syncthreads();
+  uint32_t wpre = 0;
+  for (int w = 0; w < wave; w++) wpre += wtot[w];
+  uint32_t run = wpre + incl - sum;
+  for (int k = 0; k < 4; k++) {
+    if (base + k < nb) s.off[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 255) s.tsum[blockIdx.x] = wpre + incl;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tsum(SortDev s, uint32_t ntiles) {
+  __shared__ uint32_t wtot[4];
+  __shared__ uint32_t carry_s;
+  if (threadIdx.x == 0) carry_s = 0;
+  __syncthreads();
+  for (uint32_t c = 0; c < ntiles; c += 256) {
+    const uint32_t i = c + threadIdx.x;
+    const uint32_t v = i < ntiles ? s.tsum[i] : 0u;
+    const uint32_t incl = wave_incl_scan(v);
+    const int wave = threadIdx.x >> 6, lane = __lane_id();
+    if (lane == 63) wtot[wave] = incl;
+    __syncthreads();
+    uint32_t wpre = 0, tot = 0;
+    for (int w = 0; w < 4; w++) {
+      if (w < wave) wpre += wtot[w];
+      tot += wtot[w];
+    }
+    const uint32_t carry = carry_s;
+    if (i < ntiles) s.tsum[i] = carry + wpre + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 0) carry_s = carry + tot;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(uint32_t n, SortDev s) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = s.key[i];
+  s.perm[s.off[k] + s.tsum[k / kScanTile] + s.rank[i]] = i;
+}
+
+// Zero the accumulators of this substep's active blocks (one wave per block)
+// and reset their activation flags.
+__global__ __launch_bounds__(256) void k_clear(SortDev s, GridDev g, int q32) {
+  const uint32_t lane = __lane_id();
+  const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * 256u) >> 6;
+  const uint32_t nact = s.counters[0];
+  for (uint32_t a = wave; a < nact; a += nwaves) {
+    const uint32_t b = s.active[a];
+    const uint32_t node = (b << 6) | lane;
+    if (q32) {
+      for (int f = 0; f < 7; f++) g.accq[(size_t)f * g.nodes + node] = 0;
+    } else {
+      for (int f = 0; f < 7; f++) g.accf[(size_t)f * g.nodes + node] = 0.0f;
+    }
+    if (lane == 0) s.flag[b] = 0u;
+  }
+}
+
+// Working slot j <- canonical particle idx[j] (canonical arrays are FP32).
+__global__ __launch_bounds__(256) void k_gather_work(PBuf canon, PBuf work, const uint32_t* idx,
+                                                     uint32_t n, int storage) {
+  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t p = idx[j];
+  for (int a = 0; a < 3; a++) {
+    work.x[a][j] = canon.x[a][p];
+    work.v[a][j] = canon.v[a][p];
+  }
+  for (int k = 0; k < 9; k++) {
+    put_c(work, j, k, canon.C[k][p], storage);
+    put_f(work, j, k, canon.F[k][p], storage);
+  }
+  work.J[j] = canon.J[p];
+  work.Jp[j] = canon.Jp[p];
+  work.mass[j] = canon.mass[p];
+  work.mat[j] = canon.mat[p];
+  work.id[j] = j;
+}
+
+__global__ __launch_bounds__(256) void k_scatter_work(PBuf work, PBuf canon, const uint32_t* idx,
+                                                      uint32_t n, int storage) {
+  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t p = idx[work.id[j]];
+  for (int a = 0; a < 3; a++) {
+    canon.x[a][p] = work.x[a][j];
+    canon.v[a][p] = work.v[a][j];
+  }
+  for (int k = 0; k < 9; k++) {
+    canon.C[k][p] = get_c(work, j, k, storage);
+    canon.F[k][p] = get_f(work, j, k, storage);
+  }
+  canon.J[p] = work.J[j];
+  canon.Jp[p] = work.Jp[j];
+}
+
+__global__ __launch_bounds__(256) void k_fill(float* p, float v, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_grid_readback(const SceneDev* __restrict__ scd,
+                                                       int scene, SortDev s, GridDev g, int q32,
+                                                       float* m, float* p3, float* f3, float* v3,
+                                                       long long* q7) {
+  const SceneDev S = scd[scene];
+  const uint32_t lane = __lane_id();
+  const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * 256u) >> 6;
+  const uint32_t nact = s.counters[0];
+  const uint32_t nscene_blocks = (uint32_t)(S.nb[0] * S.nb[1] * S.nb[2]);
+  for (uint32_t a = wave; a < nact; a += nwaves) {
+    const uint32_t b = s.active[a];
+    if (b < S.block_off || b >= S.block_off + nscene_blocks) continue;
+    const uint32_t lb = b - S.block_off;
+    const int bx = (int)(lb % (uint32_t)S.nb[0]);
+    const int by = (int)((lb / (uint32_t)S.nb[0]) % (uint32_t)S.nb[1]);
+    const int bz = (int)(lb / (uint32_t)(S.nb[0] * S.nb[1]));
+    const int gx = bx * 4 + (int)(lane & 3), gy = by * 4 + (int)((lane >> 2) & 3),
+              gz = bz * 4 + (int)(lane >> 4);
+    if (gx >= S.n[0] || gy >= S.n[1] || gz >= S.n[2]) continue;
+    const size_t d = (size_t)gx + (size_t)gy * S.n[0] + (size_t)gz * S.n[0] * S.n[1];
+    const uint32_t node = (b << 6) | lane;
+    const float4 gv = g.gv[node];
+    if (q32) {
+      float acc[7];
+      for (int f = 0; f < 7; f++) {
+        long long r = g.accq[(size_t)f * g.nodes + node];
+        if (q7) q7[7 * d + f] = r;
+        acc[f] = fp_to_f32(r);
+      }
+      if (m) m[d] = acc[0];
+      if (p3) for (int k = 0; k < 3; k++) p3[3 * d + k] = acc[1 + k];
+      if (f3) for (int k = 0; k < 3; k++) f3[3 * d + k] = acc[4 + k];
+    } else {
+      if (m) m[d] = g.accf[node];
+      if (p3) for (int k = 0; k < 3; k++) p3[3 * d + k] = g.accf[(size_t)(1 + k) * g.nodes + node];
+      if (f3) for (int k = 0; k < 3; k++) f3[3 * d + k] = g.accf[(size_t)(4 + k) * g.nodes + node];
+    }
+    if (v3) {
+      v3[3 * d + 0] = gv.x;
+      v3[3 * d + 1] = gv.y;
+      v3[3 * d + 2] = gv.z;
+    }
+  }
+}
+
+// FP64 partial sums per workgroup, then one workgroup adds the partials in a
+// fixed order: deterministic reductions.
+constexpr int kRedWgs = 512;
+__global__ __launch_bounds__(256) void k_conservation_partial(PBuf cur, uint32_t n, double* part) {
+  __shared__ double sh[5][256];
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    double m = cur.mass[i];
+    double vx = cur.v[0][i], vy = cur.v[1][i], vz = cur.v[2][i];
+    acc[0] += m;
+    acc[1] += m * vx;
+    acc[2] += m * vy;
+    acc[3] += m * vz;
+    acc[4] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+  }
+  for (int k = 0; k < 5; k++) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 5; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) part[blockIdx.x * 5 + threadIdx.x] = sh[threadIdx.x][0];
+}
+__global__ void k_conservation_final(const double* part, int nparts, double* out5) {
+  if (threadIdx.x < 5) {
+    double s = 0;
+    for (int b = 0; b < nparts; b++) s += part[b * 5 + threadIdx.x];
+    out5[threadIdx.x] = s;
+  }
+}
+
+inline unsigned grid_for(uint32_t n, unsigned wg) { return (unsigned)((n + wg - 1) / wg); }
+
+}  // namespace
+
+void launch_step_begin(hipStream_t st, SortDev s) { k_step_begin<<<1, 64, 0, st>>>(s); }
+void launch_count(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, uint32_t n, SortDev s) {
+  if (n) k_count<<<grid_for(n, 256), 256, 0, st>>>(cur, sc, ns, n, s);
+}
+void launch_scan(hipStream_t st, SortDev s, uint32_t nb) {
+  const uint32_t tiles = (nb + kScanTile - 1) / kScanTile;
+  k_scan_tiles<<<tiles, 256, 0, st>>>(s, nb);
+  k_scan_tsum<<<1, 256, 0, st>>>(s, tiles);
+}
+void launch_scatter(hipStream_t st, uint32_t n, SortDev s) {
+  if (n) k_scatter<<<grid_for(n, 256), 256, 0, st>>>(n, s);
+}
+void launch_clear(hipStream_t st, SortDev s, GridDev g, int q32, int upd_wgs) {
+  k_clear<<<upd_wgs, 256, 0, st>>>(s, g, q32);
+}
+void launch_gather_work(hipStream_t st, PBuf canon, PBuf work, const uint32_t* idx, uint32_t n,
+                        int storage) {
+  if (n) k_gather_work<<<grid_for(n, 256), 256, 0, st>>>(canon, work, idx, n, storage);
+}
+void launch_scatter_work(hipStream_t st, PBuf work, PBuf canon, const uint32_t* idx, uint32_t n,
+                         int storage) {
+  if (n) k_scatter_work<<<grid_for(n, 256), 256, 0, st>>>(work, canon, idx, n, storage);
+}
+void launch_fill(hipStream_t st, float* p, float v, uint32_t n) {
+  if (n) k_fill<<<grid_for(n, 256), 256, 0, st>>>(p, v, n);
+}
+void launch_grid_readback(hipStream_t st, const SceneDev* sc_d, int scene, SceneDev, SortDev s,
+                          GridDev g, int q32, float* m, float* p3, float* f3, float* v3,
+                          long long* q7, int upd_wgs) {
+  k_grid_readback<<<upd_wgs, 256, 0, st>>>(sc_d, scene, s, g, q32, m, p3, f3, v3, q7);
+}
+void launch_conservation(hipStream_t st, PBuf cur, uint32_t n, double* out5) {
+  // out5 must have room for 5 + 5 * kRedWgs doubles (partials follow the result).
+  k_conservation_partial<<<kRedWgs, 256, 0, st>>>(cur, n, out5 + 5);
+  k_conservation_final<<<1, 64, 0, st>>>(out5 + 5, kRedWgs, out5);
+}
+
+}  // namespace aemh

@@ -1,0 +1,178 @@
+/* aemh.h -- C ABI of the MI355X (gfx950) MLS-MPM step for AE-ENGINE.
+ *
+ * Drop-in for the CPU hot path the engine runs every tick:
+ *   ae_particle::mpm_solver::mpm_step_parallel_with_grid
+ *       (ae_particle/src/mpm_solver.rs:1063-1084)
+ *   ae_particle::mpm_solver::mpm_step_velocity_only_parallel
+ *       (ae_particle/src/mpm_solver.rs:1337-1351)
+ * following the FFI pattern of ae_cpp_kernel (ae_cpp_kernel/src/lib.rs:21-32,
+ * cpp/simd_kernels.cpp:20): extern "C", plain pointers + sizes, caller-owned
+ * host buffers borrowed only for the duration of a call.  Unlike the
+ * template's `void` returns every entry point returns an aemh_status and
+ * never aborts; aemh_last_error() gives the message.
+ *
+ * Particle data crosses the boundary in the reference's MpssBuffer layout
+ * (ae_particle/src/mpss.rs:104-157): pos/vel as [n][3] f32, strain (F) and C
+ * as [n][9] f32 row-major, jacobian/mass as [n] f32, material_idx as [n] u16.
+ * State stays resident in HBM between calls; downloads return the original
+ * slot order (the particle_indices semantics of the reference).
+ *
+ * Threading: one host thread per context at a time (the reference takes
+ * &mut MpssBuffer).  All calls are synchronous on return.
+ */
+#ifndef AEMH_H_
+#define AEMH_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct aemh_ctx aemh_ctx;
+
+typedef enum {
+  AEMH_OK = 0,
+  AEMH_ERR_ARG = 1,        /* bad argument / size / null pointer          */
+  AEMH_ERR_HIP = 2,        /* HIP runtime error (message has the name)    */
+  AEMH_ERR_OOM = 3,        /* device allocation failed                    */
+  AEMH_ERR_STATE = 4,      /* call order (e.g. step before upload)        */
+  AEMH_ERR_OVERFLOW = 5,   /* Q32.32 accumulation could have wrapped      */
+  AEMH_ERR_NO_DEVICE = 6   /* no gfx950 device / HIP not usable           */
+} aemh_status;
+
+/* Constitutive model ids: the Rust enum order of ConstitutiveModel
+ * (ae_particle/src/constitutive.rs:11-27); AEMH_MODEL_SNOW is an addition
+ * (Stomakhin 2013 clamp + J_p hardening, no reference counterpart). */
+enum {
+  AEMH_MODEL_NEO_HOOKEAN = 0,
+  AEMH_MODEL_DRUCKER_PRAGER = 1,
+  AEMH_MODEL_VON_MISES = 2,
+  AEMH_MODEL_NEWTONIAN_FLUID = 3,
+  AEMH_MODEL_COROTATED = 4,
+  AEMH_MODEL_FIXED_COROTATED = 5,
+  AEMH_MODEL_RIGID = 6,
+  AEMH_MODEL_SNOW = 7
+};
+
+/* P2G accumulation: FP32 atomics, or Q32.32 int64 (FixedPoint format of
+ * ae_physics/src/fixed_point.rs:4-38) -- the deterministic mode. */
+enum { AEMH_ACCUM_F32 = 0, AEMH_ACCUM_Q32_32 = 1 };
+/* G2P x/z clamp: reference serial path [0,(n-1)dx] (mpm_solver.rs:1002-1015),
+ * parallel path [o, o+(n-1)dx] (mpm_solver.rs:1287-1290), or AUTO = the
+ * reference's own switch at 50,000 indices (mpm_solver.rs:1077). */
+enum { AEMH_CLAMP_AUTO = 0, AEMH_CLAMP_SERIAL = 1, AEMH_CLAMP_WINDOW = 2 };
+/* Particle storage of C and F in HBM (compute is always FP32). */
+enum { AEMH_STORE_F32 = 0, AEMH_STORE_BF16 = 1, AEMH_STORE_F16 = 2 };
+
+/* MpmConfig (ae_particle/src/mpm_solver.rs:44-70) plus the options of this
+ * implementation.  aemh_config_default() gives the reference defaults:
+ * substeps 1, gravity (0,-9.81,0), damping 0.995, no wall margin. */
+typedef struct {
+  uint32_t substeps;
+  float gravity[3];
+  float damping;
+  uint32_t bc_margin;      /* 0: floor rule only (mpm_solver.rs:131-136);
+                              3: plus the 3-cell wall margin of
+                              ae_compute/src/mpm_compute.rs:343-389     */
+  float bc_restitution;
+  uint32_t accum;          /* AEMH_ACCUM_*                               */
+  uint32_t clamp_mode;     /* AEMH_CLAMP_*                               */
+  uint32_t storage;        /* AEMH_STORE_*                               */
+  uint32_t reserved[4];
+} aemh_config;
+
+/* One independent scene of a batched step: a contiguous range of the active
+ * particle set, its own grid (the engine's near/mid tier grids with their
+ * moving-window origins, ae_engine/src/managers/simulation.rs:541-607). */
+typedef struct {
+  uint32_t first;          /* first position in the active index list     */
+  uint32_t count;          /* particles of this scene                      */
+  uint32_t nx, ny, nz;     /* grid nodes                                   */
+  float dx;
+  float origin[3];
+  float dt;                /* substep dt of this scene                     */
+  uint32_t velocity_only;  /* 1: mpm_substep_velocity_only semantics       */
+  uint32_t clamp_mode;     /* AEMH_CLAMP_*                                 */
+  uint32_t reserved;
+} aemh_scene;
+
+void aemh_config_default(aemh_config* cfg);
+const char* aemh_version(void);
+
+/* Lifecycle ------------------------------------------------------------ */
+aemh_status aemh_create(aemh_ctx** out, int device);
+aemh_status aemh_destroy(aemh_ctx* ctx);
+const char* aemh_last_error(const aemh_ctx* ctx);
+
+/* Configuration -------------------------------------------------------- */
+aemh_status aemh_set_config(aemh_ctx* ctx, const aemh_config* cfg);
+/* Grid (ae_particle/src/mpm_solver.rs:73-103, origin :86). */
+aemh_status aemh_set_grid(aemh_ctx* ctx, uint32_t nx, uint32_t ny, uint32_t nz,
+                          float dx, const float origin[3]);
+/* material_table / model_table (mpm_solver.rs:1067-1068): params is n x 7 f32
+ * in MaterialParams field order (constitutive.rs:31-39), models n bytes.
+ * ext (may be NULL) is n x 3 f32 (theta_c, theta_s, xi) for AEMH_MODEL_SNOW. */
+aemh_status aemh_set_materials(aemh_ctx* ctx, uint32_t n, const float* params7,
+                               const uint8_t* models, const float* ext3);
+
+/* Particle state (MpssBuffer arrays, mpss.rs:104-157) ------------------- */
+aemh_status aemh_upload_particles(aemh_ctx* ctx, uint32_t n, const float* pos3,
+                                  const float* vel3, const float* C9,
+                                  const float* F9, const float* J,
+                                  const float* mass, const uint16_t* mat);
+/* Plastic volume ratio J_p of the snow model (n floats; default 1). */
+aemh_status aemh_upload_plastic_j(aemh_ctx* ctx, const float* Jp);
+aemh_status aemh_download_particles(aemh_ctx* ctx, float* pos3, float* vel3,
+                                    float* C9, float* F9, float* J);
+aemh_status aemh_download_plastic_j(aemh_ctx* ctx, float* Jp);
+/* The active set: particle_indices of mpm_step_parallel_with_grid. */
+aemh_status aemh_set_indices(aemh_ctx* ctx, const uint32_t* idx, uint32_t n_idx);
+
+/* Steps ----------------------------------------------------------------- */
+/* mpm_step_parallel_with_grid (mpm_solver.rs:1063-1084): substeps of
+ * total_dt/substeps over the active set on the context grid. */
+aemh_status aemh_step(aemh_ctx* ctx, float total_dt);
+/* Same, with the index list given per call as the reference does; the
+ * active set is re-gathered only when the list changes. */
+aemh_status aemh_mpm_step_parallel_with_grid(aemh_ctx* ctx, const uint32_t* idx,
+                                             uint32_t n_idx, float total_dt);
+/* mpm_step_velocity_only_parallel (mpm_solver.rs:1337-1351). */
+aemh_status aemh_step_velocity_only(aemh_ctx* ctx, float total_dt);
+
+/* Batched scenes: every scene advances in one launch per phase, the whole
+ * substep captured in a hipGraph and replayed n_substeps times. */
+aemh_status aemh_set_scenes(aemh_ctx* ctx, const aemh_scene* scenes, uint32_t n);
+aemh_status aemh_batch_step(aemh_ctx* ctx, uint32_t n_substeps);
+/* Tunes P2G/G2P launch shapes on the current scenes by timing candidates;
+ * the choice is kept by the context.  Called once by the first batch step. */
+aemh_status aemh_autotune(aemh_ctx* ctx);
+
+/* Grid readback for parity checks (reference layout, x-fastest index
+ * i + j nx + k nx ny, mpm_solver.rs:157-159) of scene `scene`, as left by
+ * the last substep: after P2G (mass, momentum, force) and after the update
+ * (velocity).  NULL outputs are skipped. */
+aemh_status aemh_download_grid(aemh_ctx* ctx, uint32_t scene, float* mass,
+                               float* momentum3, float* force3, float* vel3);
+/* Q32.32 raw sums (7 per node: m, p0..2, f0..2) of the last substep. */
+aemh_status aemh_download_grid_q32(aemh_ctx* ctx, uint32_t scene, int64_t* q7);
+
+/* FP64 reductions over the active set: total mass, momentum[3], kinetic
+ * energy (ae_physics/src/mpm.rs:752-771 diagnostics, in FP64). out[5]. */
+aemh_status aemh_conservation_f64(aemh_ctx* ctx, double* out5);
+
+/* Per-kernel HIP-event timing on the context stream (eager launches). */
+aemh_status aemh_profile_enable(aemh_ctx* ctx, int on);
+aemh_status aemh_profile_get(aemh_ctx* ctx, const char* kernel, double* avg_ms,
+                             uint64_t* count);
+aemh_status aemh_profile_reset(aemh_ctx* ctx);
+
+/* Deterministic scene generators (xoshiro256** seeded by splitmix64), host
+ * only, for the benchmark workloads.  Fill caller arrays of n particles. */
+void aemh_scene_uniform_box(uint64_t seed, uint32_t n, const float lo[3],
+                            const float hi[3], float* pos3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AEMH_H_ */
