@@ -73,7 +73,8 @@ EXPORTS = [
     "aemh_set_indices", "aemh_step", "aemh_mpm_step_parallel_with_grid",
     "aemh_step_velocity_only", "aemh_set_scenes", "aemh_batch_step", "aemh_autotune",
     "aemh_download_grid", "aemh_download_grid_q32", "aemh_conservation_f64",
-    "aemh_profile_enable", "aemh_profile_get", "aemh_profile_reset", "aemh_scene_uniform_box",
+    "aemh_last_substep_info", "aemh_profile_enable", "aemh_profile_get", "aemh_profile_reset",
+    "aemh_scene_uniform_box",
 ]
 
 _lib = None
@@ -114,6 +115,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.aemh_download_grid.argtypes = [P, u32, fp, fp, fp, fp]
     lib.aemh_download_grid_q32.argtypes = [P, u32, C.POINTER(C.c_int64)]
     lib.aemh_conservation_f64.argtypes = [P, C.POINTER(C.c_double)]
+    lib.aemh_last_substep_info.argtypes = [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     lib.aemh_profile_enable.argtypes = [P, C.c_int]
     lib.aemh_profile_get.argtypes = [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
     lib.aemh_profile_reset.argtypes = [P]
@@ -289,6 +291,13 @@ class Engine:
         out = (C.c_double * 5)()
         self._check(self._lib.aemh_conservation_f64(self._h, out))
         return np.array(out[:], np.float64)
+
+    def last_substep_info(self):
+        """(active 4^3 node blocks, particles) of the last substep."""
+        nb = C.c_uint32()
+        npart = C.c_uint32()
+        self._check(self._lib.aemh_last_substep_info(self._h, C.byref(nb), C.byref(npart)))
+        return nb.value, npart.value
 
     # -- profiling ----------------------------------------------------------------
     def profile(self, on=True):

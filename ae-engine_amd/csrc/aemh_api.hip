@@ -90,7 +90,7 @@ struct aemh_ctx {
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> ev_pool;
   // pinned readback word for overflow checks
-  unsigned long long* qmax_h = nullptr;
+  uint32_t* flag_h = nullptr;
 };
 
 namespace {
@@ -255,8 +255,8 @@ aemh_status ensure_grid_sort(aemh_ctx* c) {
     const size_t nodes = (size_t)nb * kBlockNodes;
     if (nodes >= (1ull << 32)) return fail(c, AEMH_ERR_ARG, "grid too large: %zu nodes", nodes);
     if (q32) {
-      HIPTRY(c, hipMalloc(&c->grid.accq, 7 * nodes * sizeof(long long)));
-      HIPTRY(c, hipMemset(c->grid.accq, 0, 7 * nodes * sizeof(long long)));
+      HIPTRY(c, hipMalloc(&c->grid.accq, kDetFields * nodes * sizeof(long long)));
+      HIPTRY(c, hipMemset(c->grid.accq, 0, kDetFields * nodes * sizeof(long long)));
     } else {
       HIPTRY(c, hipMalloc(&c->grid.accf, 7 * nodes * sizeof(float)));
       HIPTRY(c, hipMemset(c->grid.accf, 0, 7 * nodes * sizeof(float)));
@@ -287,8 +287,6 @@ aemh_status ensure_grid_sort(aemh_ctx* c) {
   if (!c->sort.counters) {
     HIPTRY(c, hipMalloc(&c->sort.counters, 64));
     HIPTRY(c, hipMemset(c->sort.counters, 0, 64));
-    HIPTRY(c, hipMalloc(&c->sort.qmax, 8));
-    HIPTRY(c, hipMemset(c->sort.qmax, 0, 8));
   }
   if (n > c->sort_n_cap || !c->sort.key) {
     for (void* p : {(void*)c->sort.key, (void*)c->sort.rank, (void*)c->sort.perm})
@@ -386,7 +384,7 @@ void enqueue_substep(aemh_ctx* c, int cur) {
   hipStream_t st = c->stream;
   timed(c, "step_begin", [&] { launch_step_begin(st, c->sort); });
   timed(c, "count", [&] { launch_count(st, a, c->scd_d, ns, n, c->sort); });
-  timed(c, "scan", [&] { launch_scan(st, c->sort, c->nb_total); });
+  timed(c, "scan", [&] { launch_scan(st, c->scd_d, ns, c->sort, c->nb_total); });
   timed(c, "scatter", [&] { launch_scatter(st, n, c->sort); });
   timed(c, "clear", [&] { launch_clear(st, c->sort, c->grid, q32, c->shape.upd_wgs); });
   if (q32) {
@@ -426,15 +424,18 @@ aemh_status build_graphs(aemh_ctx* c) {
 
 aemh_status check_overflow(aemh_ctx* c) {
   if (c->cfg.accum != AEMH_ACCUM_Q32_32) return AEMH_OK;
-  HIPTRY(c, hipMemcpyAsync(c->qmax_h, c->sort.qmax, 8, hipMemcpyDeviceToHost, c->stream));
+  // The grid update flags any node whose sum of |contribution| reached 2^63
+  // raw: only then could an int64 partial sum have wrapped in some order.
+  HIPTRY(c, hipMemcpyAsync(c->flag_h, c->sort.counters + kCntOverflow, 4, hipMemcpyDeviceToHost,
+                           c->stream));
   HIPTRY(c, hipStreamSynchronize(c->stream));
-  // A node receives at most one contribution per particle and field, so
-  // |sum| <= n * max|contribution| bounds every Q32.32 node sum.
-  unsigned __int128 bound = (unsigned __int128)(*c->qmax_h) * (unsigned __int128)c->n_work;
-  if (bound >= ((unsigned __int128)1 << 63))
+  if (*c->flag_h) {
+    HIPTRY(c, hipMemsetAsync(c->sort.counters + kCntOverflow, 0, 4, c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
     return fail(c, AEMH_ERR_OVERFLOW,
-                "Q32.32 accumulation may have wrapped (max |contribution| %llu raw, %u particles)",
-                *c->qmax_h, c->n_work);
+                "Q32.32 accumulation out of range: a node's sum of |contribution| reached 2^31 "
+                "(FixedPoint would saturate in an order-dependent way)");
+  }
   return AEMH_OK;
 }
 
@@ -581,7 +582,7 @@ aemh_status aemh_create(aemh_ctx** out, int device) {
   c->device = device;
   aemh_config_default(&c->cfg);
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
-      hipHostMalloc((void**)&c->qmax_h, 8, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->flag_h, 8, hipHostMallocDefault) != hipSuccess) {
     delete c;
     return AEMH_ERR_HIP;
   }
@@ -604,9 +605,9 @@ aemh_status aemh_destroy(aemh_ctx* c) {
                   (void*)c->grid.accq, (void*)c->grid.gv, (void*)c->sort.key, (void*)c->sort.rank,
                   (void*)c->sort.perm, (void*)c->sort.cnt, (void*)c->sort.off, (void*)c->sort.tsum,
                   (void*)c->sort.flag, (void*)c->sort.active, (void*)c->sort.counters,
-                  (void*)c->sort.qmax})
+                  })
     if (p) hipFree(p);
-  if (c->qmax_h) hipHostFree(c->qmax_h);
+  if (c->flag_h) hipHostFree(c->flag_h);
   hipStreamDestroy(c->stream);
   delete c;
   return AEMH_OK;
@@ -702,7 +703,7 @@ aemh_status aemh_upload_particles(aemh_ctx* c, uint32_t n, const float* pos3, co
   CHECK_LAUNCH(c);
   c->work_valid = false;
   c->work_dirty = false;
-  if (c->sort.qmax) HIPTRY(c, hipMemset(c->sort.qmax, 0, 8));
+  if (c->sort.counters) HIPTRY(c, hipMemset(c->sort.counters, 0, 64));
   return AEMH_OK;
 }
 
@@ -892,6 +893,17 @@ aemh_status aemh_conservation_f64(aemh_ctx* c, double* out5) {
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   hipFree(tmp);
   if (e != hipSuccess) return fail(c, AEMH_ERR_HIP, "reduction: %s", hipGetErrorName(e));
+  return AEMH_OK;
+}
+
+aemh_status aemh_last_substep_info(aemh_ctx* c, uint32_t* active_blocks, uint32_t* particles) {
+  if (!c) return AEMH_ERR_ARG;
+  hipSetDevice(c->device);
+  uint32_t nact = 0;
+  if (c->sort.counters)
+    HIPTRY(c, hipMemcpy(&nact, c->sort.counters + kCntActive, 4, hipMemcpyDeviceToHost));
+  if (active_blocks) *active_blocks = nact;
+  if (particles) *particles = c->n_work;
   return AEMH_OK;
 }
 

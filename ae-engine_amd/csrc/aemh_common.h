@@ -71,10 +71,16 @@ struct PBuf {
 // Grid accumulators and the updated velocity read by G2P.
 struct GridDev {
   float* accf;      // [7][nodes] m, p0..2, f0..2 (FP32 mode)
-  long long* accq;  // [7][nodes] Q32.32 raw sums (deterministic mode)
+  long long* accq;  // [8][nodes] Q32.32 raw sums (deterministic mode); field 7
+                    // bounds sum |contribution| in units of 2^32 raw
   float4* gv;       // [nodes] (v.xyz, m) after the grid update
   uint32_t nodes;
 };
+constexpr int kDetFields = 8;
+// counters[0]: active blocks of the substep; counters[4]: sticky Q32.32
+// overflow flag (a node's sum |contribution| reached 2^63 raw).
+constexpr int kCntActive = 0;
+constexpr int kCntOverflow = 4;
 
 // Per-substep bookkeeping buffers.
 struct SortDev {
@@ -86,8 +92,7 @@ struct SortDev {
   uint32_t* tsum;       // [tiles] tile sums, scanned in place
   uint32_t* flag;       // [NB] node block active this substep
   uint32_t* active;     // [NB] list of active node blocks
-  uint32_t* counters;   // [8] 0: active blocks, 1: overflow flag
-  unsigned long long* qmax;  // max |raw| of a Q32.32 contribution
+  uint32_t* counters;   // [8], see kCnt*
 };
 
 constexpr int kScanTile = 1024;

@@ -1,10 +1,9 @@
 // aemh_dev.h -- device-side types and per-particle math of the MLS-MPM step.
 //
-// Included by every HIP translation unit.  The same source is compiled twice:
-// k_fast.hip with -ffp-contract=fast (FP32 mode) and k_det.hip with
-// -ffp-contract=off (deterministic Q32.32 mode, whose per-particle
-// arithmetic must reproduce the reference's rustc evaluation order exactly;
-// rustc never fuses multiply-adds).
+// Included by every HIP translation unit.  The kernels are compiled twice,
+// k_fast.hip (FP32 mode) and k_det.hip (deterministic Q32.32 mode), both with
+// -ffp-contract=off: per-particle arithmetic keeps the reference's rustc
+// evaluation order (rustc never fuses multiply-adds).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

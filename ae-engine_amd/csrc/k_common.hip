@@ -8,14 +8,14 @@
 namespace aemh {
 namespace {
 
+// Per-substep counters (not the sticky overflow flag).
 __global__ void k_step_begin(SortDev s) {
-  if (threadIdx.x < 8) s.counters[threadIdx.x] = 0;
+  if (threadIdx.x < kCntOverflow) s.counters[threadIdx.x] = 0;
 }
 
 // Key + rank: every working slot computes the global block of its clamped
 // base cell; runs of equal keys inside a wave take one atomic on the block
-// counter, so nearly sorted input costs about one atomic per wave.  Each run
-// head also activates the 2x2x2 node blocks its 6^3 halo can touch.
+// counter, so nearly sorted input costs about one atomic per wave.
 __global__ __launch_bounds__(256) void k_count(PBuf cur, const SceneDev* __restrict__ sc, int ns,
                                                uint32_t n, SortDev s) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -23,14 +23,10 @@ __global__ __launch_bounds__(256) void k_count(PBuf cur, const SceneDev* __restr
   if (__ballot(valid) == 0) return;
   const int lane = __lane_id();
   uint32_t key = 0xffffffffu;
-  int bc[3] = {0, 0, 0};
-  int sid = 0;
   if (valid) {
-    sid = scene_of(sc, ns, i);
-    const SceneDev& S = sc[sid];
-    Stencil st = make_stencil(S, cur.x[0][i], cur.x[1][i], cur.x[2][i]);
-    for (int a = 0; a < 3; a++) bc[a] = min(st.g[a], S.n[a] - 1) >> 2;
-    key = S.block_off + (uint32_t)((bc[2] * S.nb[1] + bc[1]) * S.nb[0] + bc[0]);
+    const SceneDev S = sc[scene_of(sc, ns, i)];
+    const Stencil st = make_stencil(S, cur.x[0][i], cur.x[1][i], cur.x[2][i]);
+    key = block_key(S, st);
     s.key[i] = key;
   }
   const uint32_t prev = __shfl_up(key, 1, 64);
@@ -43,19 +39,6 @@ __global__ __launch_bounds__(256) void k_count(PBuf cur, const SceneDev* __restr
     const unsigned long long above = heads & ~le;
     const int next = above ? (__ffsll((long long)above) - 1) : nvalid;
     old = atomicAdd(&s.cnt[key], (uint32_t)(next - lane));
-    const SceneDev& S = sc[sid];
-    for (int dz = 0; dz < 2; dz++)
-      for (int dy = 0; dy < 2; dy++)
-        for (int dx = 0; dx < 2; dx++) {
-          int bx = bc[0] + dx, by = bc[1] + dy, bz = bc[2] + dz;
-          if (bx >= S.nb[0] || by >= S.nb[1] || bz >= S.nb[2]) continue;
-          uint32_t b = S.block_off + (uint32_t)((bz * S.nb[1] + by) * S.nb[0] + bx);
-          if (__atomic_load_n(&s.flag[b], __ATOMIC_RELAXED) == 0u &&
-              atomicExch(&s.flag[b], 1u) == 0u) {
-            uint32_t slot = atomicAdd(&s.counters[0], 1u);
-            s.active[slot] = b;
-          }
-        }
   }
   const unsigned long long mine = valid ? (heads & le) : 1ull;
   const int h = 63 - __clzll((long long)mine);
@@ -63,15 +46,35 @@ __global__ __launch_bounds__(256) void k_count(PBuf cur, const SceneDev* __restr
   if (valid) s.rank[i] = base + (uint32_t)(lane - h);
 }
 
+// Every non-empty base block activates the 2x2x2 node blocks its 6^3 halo
+// can touch (idempotent plain stores, no atomics).
+__device__ __forceinline__ void activate_halo(const SceneDev* __restrict__ sc, int ns, SortDev& s,
+                                              uint32_t b) {
+  const SceneDev S = sc[scene_of_block(sc, ns, b)];
+  const uint32_t lb = b - S.block_off;
+  const int bx = (int)(lb % (uint32_t)S.nb[0]);
+  const int by = (int)((lb / (uint32_t)S.nb[0]) % (uint32_t)S.nb[1]);
+  const int bz = (int)(lb / (uint32_t)(S.nb[0] * S.nb[1]));
+  for (int dz = 0; dz < 2; dz++)
+    for (int dy = 0; dy < 2; dy++)
+      for (int dx = 0; dx < 2; dx++) {
+        const int x = bx + dx, y = by + dy, z = bz + dz;
+        if (x >= S.nb[0] || y >= S.nb[1] || z >= S.nb[2]) continue;
+        s.flag[S.block_off + (uint32_t)((z * S.nb[1] + y) * S.nb[0] + x)] = 1u;
+      }
+}
+
 // Exclusive scan of the per-block counts, tile of 1024 per workgroup; zeroes
-// the counts for the next substep.
-__global__ __launch_bounds__(256) void k_scan_tiles(SortDev s, uint32_t nb) {
+// the counts for the next substep and activates the halos of non-empty blocks.
+__global__ __launch_bounds__(256) void k_scan_tiles(const SceneDev* __restrict__ sc, int ns,
+                                                    SortDev s, uint32_t nb) {
   __shared__ uint32_t wtot[4];
   const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 4u;
   uint32_t v[4];
   for (int k = 0; k < 4; k++) {
     v[k] = (base + k < nb) ? s.cnt[base + k] : 0u;
     if (base + k < nb) s.cnt[base + k] = 0u;
+    if (v[k]) activate_halo(sc, ns, s, base + k);
   }
   const uint32_t sum = v[0] + v[1] + v[2] + v[3];
   const uint32_t incl = wave_incl_scan(sum);
@@ -113,6 +116,25 @@ __global__ __launch_bounds__(256) void k_scan_tsum(SortDev s, uint32_t ntiles) {
   }
 }
 
+// Compact the activation flags into the active-block list (one atomic per
+// wave) and clear them for the next substep.
+__global__ __launch_bounds__(256) void k_compact_active(SortDev s, uint32_t nb) {
+  const uint32_t b = blockIdx.x * 256u + threadIdx.x;
+  const bool on = b < nb && s.flag[b] != 0u;
+  const unsigned long long m = __ballot(on);
+  if (m == 0) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(&s.counters[0], (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if (on) {
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    s.active[base + (uint32_t)__popcll(m & lt)] = b;
+    s.flag[b] = 0u;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_scatter(uint32_t n, SortDev s) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= n) return;
@@ -120,8 +142,7 @@ __global__ __launch_bounds__(256) void k_scatter(uint32_t n, SortDev s) {
   s.perm[s.off[k] + s.tsum[k / kScanTile] + s.rank[i]] = i;
 }
 
-// Zero the accumulators of this substep's active blocks (one wave per block)
-// and reset their activation flags.
+// Zero the accumulators of this substep's active blocks (one wave per block).
 __global__ __launch_bounds__(256) void k_clear(SortDev s, GridDev g, int q32) {
   const uint32_t lane = __lane_id();
   const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
@@ -131,11 +152,10 @@ __global__ __launch_bounds__(256) void k_clear(SortDev s, GridDev g, int q32) {
     const uint32_t b = s.active[a];
     const uint32_t node = (b << 6) | lane;
     if (q32) {
-      for (int f = 0; f < 7; f++) g.accq[(size_t)f * g.nodes + node] = 0;
+      for (int f = 0; f < kDetFields; f++) g.accq[(size_t)f * g.nodes + node] = 0;
     } else {
       for (int f = 0; f < 7; f++) g.accf[(size_t)f * g.nodes + node] = 0.0f;
     }
-    if (lane == 0) s.flag[b] = 0u;
   }
 }
 
@@ -268,10 +288,11 @@ void launch_step_begin(hipStream_t st, SortDev s) { k_step_begin<<<1, 64, 0, st>
 void launch_count(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, uint32_t n, SortDev s) {
   if (n) k_count<<<grid_for(n, 256), 256, 0, st>>>(cur, sc, ns, n, s);
 }
-void launch_scan(hipStream_t st, SortDev s, uint32_t nb) {
+void launch_scan(hipStream_t st, const SceneDev* sc, int ns, SortDev s, uint32_t nb) {
   const uint32_t tiles = (nb + kScanTile - 1) / kScanTile;
-  k_scan_tiles<<<tiles, 256, 0, st>>>(s, nb);
+  k_scan_tiles<<<tiles, 256, 0, st>>>(sc, ns, s, nb);
   k_scan_tsum<<<1, 256, 0, st>>>(s, tiles);
+  k_compact_active<<<grid_for(nb, 256), 256, 0, st>>>(s, nb);
 }
 void launch_scatter(hipStream_t st, uint32_t n, SortDev s) {
   if (n) k_scatter<<<grid_for(n, 256), 256, 0, st>>>(n, s);

@@ -1,14 +1,21 @@
 // kernels.inc.h -- P2G, grid update and G2P of one MLS-MPM substep.
 //
-// Compiled twice: k_fast.hip defines AEMH_NS=fast, AEMH_DET=0 (FP32 atomics,
-// -ffp-contract=fast); k_det.hip defines AEMH_NS=det, AEMH_DET=1 (Q32.32 int64
-// atomics, -ffp-contract=off, reference evaluation order).
+// Compiled twice, both without multiply-add contraction (reference order):
+// k_fast.hip defines AEMH_NS=fast, AEMH_DET=0 (FP32 atomics, libm ln/exp);
+// k_det.hip defines AEMH_NS=det, AEMH_DET=1 (Q32.32 int64 atomics, portable
+// ln/exp), which makes the whole step bit-identical to a CPU run.
 //
 // Reference: ae_particle/src/mpm_solver.rs
 //   P2G          :831-903 (serial) / :1108-1197 (rayon fold/reduce)
 //   grid update  :112-139 (+ ae_compute/src/mpm_compute.rs:343-389 margin)
 //   G2P          :914-1053 (serial) / :1208-1325 (parallel)
 //   velocity-only:1353-1572
+//
+// Skipped stencil nodes (w < 1e-10, outside the grid, grid mass < 1e-12) are
+// evaluated branch-free in G2P and contribute an exact +0.0 to its sums:
+// every accumulator starts at +0.0 and an IEEE sum only reaches -0.0 from two
+// -0.0 operands, so adding +0.0 leaves each bit pattern unchanged -- the same
+// result as the reference's `continue`.
 #include "aemh_launch.h"
 #include "aemh_physics.h"
 
@@ -16,41 +23,101 @@ namespace aemh {
 namespace AEMH_NS {
 namespace {
 
-#if AEMH_DET
-using Acc = long long;
-#else
-using Acc = float;
-#endif
-
 // Up to this many distinct base blocks of one workgroup's particles are
 // staged in LDS; the rare rest scatter straight to HBM.
 constexpr int kMaxTiles = 4;
-constexpr int kTileVals = 7 * kTileNodes;
-
-__device__ __forceinline__ void acc_global(GridDev& G, int f, uint32_t node, Acc v) {
 #if AEMH_DET
-  atomicAdd((unsigned long long*)&G.accq[(size_t)f * G.nodes + node], (unsigned long long)v);
+constexpr int kTileFields = kDetFields;  // 7 sums + the magnitude bound
 #else
-  unsafeAtomicAdd(&G.accf[(size_t)f * G.nodes + node], v);
+constexpr int kTileFields = 7;
 #endif
-}
-__device__ __forceinline__ void acc_lds(Acc* t, Acc v) {
-#if AEMH_DET
-  atomicAdd((unsigned long long*)t, (unsigned long long)v);
-#else
-  atomicAdd(t, v);
-#endif
+constexpr int kTileVals = kTileFields * kTileNodes;
+
+// Per-particle P2G inputs after the constitutive update.
+struct P2GPart {
+  float wx[3], wy[3], wz[3], dwx[3], dwy[3], dwz[3];
+  float px, py, pz, vx, vy, vz, mass;
+  float c[9], sv[9];
+};
+
+// The 27-node scatter in the reference's dk-dj-di order; `put(f, di, dj, dk,
+// gx, gy, gz, value)` receives each contribution of field f (m, p0..2,
+// f0..2).  Full MPM multiplies (wx wy) wz, the velocity-only tier wx (wy wz).
+template <bool kFull, class Put>
+__device__ __forceinline__ void p2g_nodes(const int n0, const int n1, const int n2, const float dx,
+                                          const float o0, const float o1, const float o2,
+                                          const Stencil& st, const P2GPart& q, Put&& put) {
+#pragma unroll
+  for (int dk = 0; dk < 3; dk++)
+#pragma unroll
+    for (int dj = 0; dj < 3; dj++)
+#pragma unroll
+      for (int di = 0; di < 3; di++) {
+        const float w = kFull ? q.wx[di] * q.wy[dj] * q.wz[dk] : q.wx[di] * (q.wy[dj] * q.wz[dk]);
+        const int gx = st.g[0] + di, gy = st.g[1] + dj, gz = st.g[2] + dk;
+        if (w < 1e-10f || gx >= n0 || gy >= n1 || gz >= n2) continue;
+        if (kFull) {
+          const float dxg = ((float)gx + 0.5f) * dx + o0 - q.px;
+          const float dyg = ((float)gy + 0.5f) * dx + o1 - q.py;
+          const float dzg = ((float)gz + 0.5f) * dx + o2 - q.pz;
+          const float ux = q.vx + q.c[0] * dxg + q.c[1] * dyg + q.c[2] * dzg;
+          const float uy = q.vy + q.c[3] * dxg + q.c[4] * dyg + q.c[5] * dzg;
+          const float uz = q.vz + q.c[6] * dxg + q.c[7] * dyg + q.c[8] * dzg;
+          put(0, di, dj, dk, gx, gy, gz, w * q.mass);
+          put(1, di, dj, dk, gx, gy, gz, w * q.mass * ux);
+          put(2, di, dj, dk, gx, gy, gz, w * q.mass * uy);
+          put(3, di, dj, dk, gx, gy, gz, w * q.mass * uz);
+          put(4, di, dj, dk, gx, gy, gz, q.sv[0] * q.dwx[di] + q.sv[1] * q.dwy[dj] + q.sv[2] * q.dwz[dk]);
+          put(5, di, dj, dk, gx, gy, gz, q.sv[3] * q.dwx[di] + q.sv[4] * q.dwy[dj] + q.sv[5] * q.dwz[dk]);
+          put(6, di, dj, dk, gx, gy, gz, q.sv[6] * q.dwx[di] + q.sv[7] * q.dwy[dj] + q.sv[8] * q.dwz[dk]);
+        } else {
+          put(0, di, dj, dk, gx, gy, gz, w * q.mass);
+          put(1, di, dj, dk, gx, gy, gz, w * q.mass * q.vx);
+          put(2, di, dj, dk, gx, gy, gz, w * q.mass * q.vy);
+          put(3, di, dj, dk, gx, gy, gz, w * q.mass * q.vz);
+        }
+      }
 }
 
+// trunc(|x|) as an integer for |x| < 2^63, exactly: hi = floor(|x| / 2^32)
+// and lo = |x| - hi 2^32 are both exact in float (x has 24 significant bits).
+__device__ __forceinline__ unsigned long long trunc_abs_u64(float ax) {
+  const float hi = floorf(ax * 2.3283064365386963e-10f);
+  const float lo = ax - hi * 4294967296.0f;
+  return ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo;
+}
+__device__ __forceinline__ long long apply_sign(unsigned long long m, float x) {
+  return x < 0.0f ? -(long long)m : (long long)m;
+}
+
+#if AEMH_DET
+// FixedPoint::from_f32: trunc(v 2^32) with saturation, NaN -> 0.
+__device__ __forceinline__ long long to_q32(float v) {
+  if (!(fabsf(v) < 2147483648.0f)) return v != v ? 0 : (v > 0.0f ? 0x7fffffffffffffffLL : (long long)0x8000000000000000ULL);
+  return apply_sign(trunc_abs_u64(fabsf(v) * 4294967296.0f), v);
+}
+#endif
+
+// P2G.  One particle per thread over the block-sorted order.  The distinct
+// base blocks of the workgroup (usually one or two) each get a 6^3-node LDS
+// tile of int64 accumulators -- LDS integer atomics run ~30x the rate of
+// ds_add_f32 on gfx950.  The deterministic mode adds FixedPoint Q32.32 values;
+// the FP32 mode adds each value scaled by a per-workgroup power of two 2^s_g
+// (one per field group: mass, momentum, force) chosen so 256 contributions
+// cannot overflow, i.e. an exact block-floating-point sum with resolution
+// 2^-54 of the group's largest contribution.  Tiles are flushed block-
+// coalesced with one HBM atomic per non-zero node value.  The LDS and HBM
+// scatter paths are separate code so the LDS one stays ds_add.
 template <int STORE>
 __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict__ sc, int ns,
                                              const MatDev* __restrict__ mats, int nmat,
                                              uint32_t n, SortDev s, GridDev G) {
-  __shared__ Acc tile[kMaxTiles * kTileVals];
+  __shared__ long long tile[kMaxTiles * kTileVals];
   __shared__ uint32_t skey[256];
   __shared__ int tsc[kMaxTiles];
   __shared__ int torg[kMaxTiles][3];
   __shared__ uint32_t wcnt[4];
+  __shared__ int wexp[4][3];
 
   const int t = threadIdx.x;
   const int lane = __lane_id(), wave = t >> 6;
@@ -78,132 +145,215 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
   const int ntiles = (int)min(tot, (uint32_t)kMaxTiles);
 
   int sid = 0;
+  SceneDev S{};
   Stencil st = {{0, 0, 0}, {0.f, 0.f, 0.f}};
   if (valid) {
     sid = scene_of(sc, ns, j);
-    st = make_stencil(sc[sid], P.x[0][p], P.x[1][p], P.x[2][p]);
+    S = sc[sid];
+    st = make_stencil(S, P.x[0][p], P.x[1][p], P.x[2][p]);
   }
   if (head && slot < kMaxTiles) {
-    const SceneDev& S = sc[sid];
     tsc[slot] = sid;
 #pragma unroll
     for (int a = 0; a < 3; a++) torg[slot][a] = (min(st.g[a], S.n[a] - 1) >> 2) * 4;
   }
   for (int e = t; e < ntiles * kTileVals; e += 256) tile[e] = 0;
-  __syncthreads();
 
-  [[maybe_unused]] unsigned long long qmax = 0;
+  // ---- particle inputs and stress ----------------------------------------
+  const bool full = !S.vel_only;
+  P2GPart q;
+  q.px = q.py = q.pz = q.vx = q.vy = q.vz = q.mass = 0.0f;
   if (valid) {
-    const SceneDev& S = sc[sid];
-    const bool use_tile = slot < kMaxTiles;
-    int og[3] = {0, 0, 0};
-    if (use_tile) {
-      og[0] = torg[slot][0];
-      og[1] = torg[slot][1];
-      og[2] = torg[slot][2];
-    }
-    Acc* tl = tile + (use_tile ? slot : 0) * kTileVals;
-    const float px = P.x[0][p], py = P.x[1][p], pz = P.x[2][p];
-    const float vx = P.v[0][p], vy = P.v[1][p], vz = P.v[2][p];
-    const float mass = P.mass[p];
-    float wx[3], wy[3], wz[3], dwx[3], dwy[3], dwz[3];
+    const float inv_dx = S.inv_dx;
+    q.px = P.x[0][p];
+    q.py = P.x[1][p];
+    q.pz = P.x[2][p];
+    q.vx = P.v[0][p];
+    q.vy = P.v[1][p];
+    q.vz = P.v[2][p];
+    q.mass = P.mass[p];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      wx[a] = b_spline((float)a - st.fx[0]);
-      wy[a] = b_spline((float)a - st.fx[1]);
-      wz[a] = b_spline((float)a - st.fx[2]);
-      dwx[a] = b_spline_deriv((float)a - st.fx[0]) * S.inv_dx;
-      dwy[a] = b_spline_deriv((float)a - st.fx[1]) * S.inv_dx;
-      dwz[a] = b_spline_deriv((float)a - st.fx[2]) * S.inv_dx;
+      q.wx[a] = b_spline((float)a - st.fx[0]);
+      q.wy[a] = b_spline((float)a - st.fx[1]);
+      q.wz[a] = b_spline((float)a - st.fx[2]);
+      q.dwx[a] = b_spline_deriv((float)a - st.fx[0]) * inv_dx;
+      q.dwy[a] = b_spline_deriv((float)a - st.fx[1]) * inv_dx;
+      q.dwz[a] = b_spline_deriv((float)a - st.fx[2]) * inv_dx;
     }
-    float c[9], sv[9];
-    const bool full = !S.vel_only;
     if (full) {
       float F[9];
 #pragma unroll
       for (int k = 0; k < 9; k++) {
-        c[k] = get_c(P, p, k, STORE);
+        q.c[k] = get_c(P, p, k, STORE);
         F[k] = get_f(P, p, k, STORE);
       }
       const uint32_t mi = P.mat[p];
-      const MatDev& M = mats[mi < (uint32_t)nmat ? mi : (uint32_t)nmat];
+      const MatDev M = mats[mi < (uint32_t)nmat ? mi : (uint32_t)nmat];
       float stress[9];
       compute_stress(M, F, P.J[p], P.Jp[p], stress);
-      const float vol = mass / M.density;
+      const float vol = q.mass / M.density;
 #pragma unroll
-      for (int k = 0; k < 9; k++) sv[k] = vol * stress[k];
+      for (int k = 0; k < 9; k++) q.sv[k] = vol * stress[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 9; k++) q.c[k] = q.sv[k] = 0.0f;
     }
-    auto put = [&](int f, int lx, int ly, int lz, int gx, int gy, int gz, float val) {
-#if AEMH_DET
-      const Acc a = fp_from_f32(val);
-      const unsigned long long mag = a < 0 ? 0ull - (unsigned long long)a : (unsigned long long)a;
-      qmax = mag > qmax ? mag : qmax;
-      if (a == 0) return;
-#else
-      const Acc a = val;
-#endif
-      if (use_tile && (unsigned)lx < 6u && (unsigned)ly < 6u && (unsigned)lz < 6u)
-        acc_lds(&tl[f * kTileNodes + (lz * 6 + ly) * 6 + lx], a);
-      else
-        acc_global(G, f, node_addr(S, gx, gy, gz), a);
-    };
-#pragma unroll
-    for (int dk = 0; dk < 3; dk++)
-#pragma unroll
-      for (int dj = 0; dj < 3; dj++)
-#pragma unroll
-        for (int di = 0; di < 3; di++) {
-          // velocity-only tier multiplies wx * (wy * wz) (mpm_solver.rs:1425-1428)
-          const float w = full ? wx[di] * wy[dj] * wz[dk] : wx[di] * (wy[dj] * wz[dk]);
-          if (w < 1e-10f) continue;
-          const int gx = st.g[0] + di, gy = st.g[1] + dj, gz = st.g[2] + dk;
-          if (gx >= S.n[0] || gy >= S.n[1] || gz >= S.n[2]) continue;
-          const int lx = gx - og[0], ly = gy - og[1], lz = gz - og[2];
-          if (full) {
-            const float dxg = ((float)gx + 0.5f) * S.dx + S.o[0] - px;
-            const float dyg = ((float)gy + 0.5f) * S.dx + S.o[1] - py;
-            const float dzg = ((float)gz + 0.5f) * S.dx + S.o[2] - pz;
-            const float ux = vx + c[0] * dxg + c[1] * dyg + c[2] * dzg;
-            const float uy = vy + c[3] * dxg + c[4] * dyg + c[5] * dzg;
-            const float uz = vz + c[6] * dxg + c[7] * dyg + c[8] * dzg;
-            const float wm = w * mass;
-            put(0, lx, ly, lz, gx, gy, gz, wm);
-            put(1, lx, ly, lz, gx, gy, gz, w * mass * ux);
-            put(2, lx, ly, lz, gx, gy, gz, w * mass * uy);
-            put(3, lx, ly, lz, gx, gy, gz, w * mass * uz);
-            put(4, lx, ly, lz, gx, gy, gz, sv[0] * dwx[di] + sv[1] * dwy[dj] + sv[2] * dwz[dk]);
-            put(5, lx, ly, lz, gx, gy, gz, sv[3] * dwx[di] + sv[4] * dwy[dj] + sv[5] * dwz[dk]);
-            put(6, lx, ly, lz, gx, gy, gz, sv[6] * dwx[di] + sv[7] * dwy[dj] + sv[8] * dwz[dk]);
-          } else {
-            put(0, lx, ly, lz, gx, gy, gz, w * mass);
-            put(1, lx, ly, lz, gx, gy, gz, w * mass * vx);
-            put(2, lx, ly, lz, gx, gy, gz, w * mass * vy);
-            put(3, lx, ly, lz, gx, gy, gz, w * mass * vz);
-          }
-        }
   }
-#if AEMH_DET
-  // Largest |contribution| so the host can prove no node sum wrapped.
-  for (int d = 32; d > 0; d >>= 1) {
-    unsigned long long o = __shfl_xor(qmax, d, 64);
-    qmax = o > qmax ? o : qmax;
+
+#if !AEMH_DET
+  // ---- block exponents: |contribution of group g| < 2^e_g over the WG ----
+  // mass: w m <= m; momentum: |w m u_r| <= m (|v_r| + 2 dx sum_c |C_rc|);
+  // force: |sum_c sv_rc dw_c| <= inv_dx sum_c |sv_rc|   (|N'| <= 1, |d| < 2 dx).
+  // Bounds are doubled for rounding; a non-finite bound sends the particle
+  // down the HBM path, where float atomics carry NaN/inf as the reference.
+  bool finite_in = true;
+  int ex[3] = {-126, -126, -126};
+  if (valid) {
+    float bp = 0.0f, bf = 0.0f;
+    const float vv[3] = {q.vx, q.vy, q.vz};
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      bp = fmaxf(bp, fabsf(vv[r]) + 2.0f * S.dx * (fabsf(q.c[3 * r]) + fabsf(q.c[3 * r + 1]) + fabsf(q.c[3 * r + 2])));
+      bf = fmaxf(bf, S.inv_dx * (fabsf(q.sv[3 * r]) + fabsf(q.sv[3 * r + 1]) + fabsf(q.sv[3 * r + 2])));
+    }
+    const float b[3] = {2.0f * q.mass, 2.0f * q.mass * bp, 2.0f * bf};
+#pragma unroll
+    for (int g = 0; g < 3; g++) {
+      if (!(b[g] <= 3.0e38f)) finite_in = false;
+      int e = 0;
+      frexpf(b[g], &e);
+      ex[g] = b[g] > 0.0f ? e : -126;
+    }
   }
-  if (lane == 0 && qmax) atomicMax(s.qmax, qmax);
+#pragma unroll
+  for (int g = 0; g < 3; g++) {
+    int e = finite_in ? ex[g] : -126;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) e = max(e, __shfl_xor(e, d, 64));
+    if (lane == 0) wexp[wave][g] = e;
+  }
 #endif
   __syncthreads();
-  // Flush the staged halos: one global atomic per non-zero node value.
-  for (int e = t; e < ntiles * kTileVals; e += 256) {
-    const Acc v = tile[e];
+
+#if !AEMH_DET
+  int eg[3];
+  float scale[3];
+#pragma unroll
+  for (int g = 0; g < 3; g++) {
+    eg[g] = max(max(wexp[0][g], wexp[1][g]), max(wexp[2][g], wexp[3][g]));
+    eg[g] = max(eg[g], -70);
+    scale[g] = __int_as_float((127 + 54 - eg[g]) << 23);  // 2^(54 - e_g)
+  }
+#endif
+
+  if (valid) {
+    const int n0 = S.n[0], n1 = S.n[1], n2 = S.n[2];
+    const float dx = S.dx, o0 = S.o[0], o1 = S.o[1], o2 = S.o[2];
+#if AEMH_DET
+    const bool lds_path = slot < kMaxTiles;
+#else
+    const bool lds_path = slot < kMaxTiles && finite_in;
+#endif
+    if (lds_path) {
+      // A base cell inside this tile's block reaches local nodes 0..5 only.
+      const int tb = slot * kTileVals +
+                     ((st.g[2] - torg[slot][2]) * kTileSide + (st.g[1] - torg[slot][1])) * kTileSide +
+                     (st.g[0] - torg[slot][0]);
+#if AEMH_DET
+      unsigned long long magn[27];
+#pragma unroll
+      for (int k = 0; k < 27; k++) magn[k] = 0;
+      auto put = [&](int f, int di, int dj, int dk, int, int, int, float v) {
+        const long long a = to_q32(v);
+        const unsigned long long m = a < 0 ? 0ull - (unsigned long long)a : (unsigned long long)a;
+        const int k = (dk * 3 + dj) * 3 + di;
+        magn[k] = m > magn[k] ? m : magn[k];
+        atomicAdd((unsigned long long*)&tile[tb + f * kTileNodes + (dk * kTileSide + dj) * kTileSide + di],
+                  (unsigned long long)a);
+      };
+#else
+      auto put = [&](int f, int di, int dj, int dk, int, int, int, float v) {
+        const int g = f == 0 ? 0 : (f < 4 ? 1 : 2);
+        const long long a = apply_sign(trunc_abs_u64(fabsf(v) * scale[g]), v);
+        atomicAdd((unsigned long long*)&tile[tb + f * kTileNodes + (dk * kTileSide + dj) * kTileSide + di],
+                  (unsigned long long)a);
+      };
+#endif
+      if (full)
+        p2g_nodes<true>(n0, n1, n2, dx, o0, o1, o2, st, q, put);
+      else
+        p2g_nodes<false>(n0, n1, n2, dx, o0, o1, o2, st, q, put);
+#if AEMH_DET
+      // Sum-|c| bound per node in units of 2^32 raw, rounded up.
+#pragma unroll
+      for (int dk = 0; dk < 3; dk++)
+#pragma unroll
+        for (int dj = 0; dj < 3; dj++)
+#pragma unroll
+          for (int di = 0; di < 3; di++) {
+            const unsigned long long m = magn[(dk * 3 + dj) * 3 + di];
+            if (m)
+              atomicAdd((unsigned long long*)&tile[tb + 7 * kTileNodes + (dk * kTileSide + dj) * kTileSide + di],
+                        (m >> 32) + 1ull);
+          }
+#endif
+    } else {
+      const size_t nodes = G.nodes;
+      const uint32_t boff = S.block_off;
+      const int nb0 = S.nb[0], nb1 = S.nb[1];
+#if AEMH_DET
+      long long* const acc = G.accq;
+      auto put = [&](int f, int, int, int, int gx, int gy, int gz, float v) {
+        const long long a = to_q32(v);
+        const uint32_t b = boff + (uint32_t)(((gz >> 2) * nb1 + (gy >> 2)) * nb0 + (gx >> 2));
+        const uint32_t node = (b << 6) | (uint32_t)(((gz & 3) << 4) | ((gy & 3) << 2) | (gx & 3));
+        atomicAdd((unsigned long long*)(acc + (size_t)f * nodes + node), (unsigned long long)a);
+        const unsigned long long m = a < 0 ? 0ull - (unsigned long long)a : (unsigned long long)a;
+        atomicAdd((unsigned long long*)(acc + (size_t)7 * nodes + node), (m >> 32) + 1ull);
+      };
+#else
+      float* const acc = G.accf;
+      auto put = [&](int f, int, int, int, int gx, int gy, int gz, float v) {
+        const uint32_t b = boff + (uint32_t)(((gz >> 2) * nb1 + (gy >> 2)) * nb0 + (gx >> 2));
+        const uint32_t node = (b << 6) | (uint32_t)(((gz & 3) << 4) | ((gy & 3) << 2) | (gx & 3));
+        unsafeAtomicAdd(acc + (size_t)f * nodes + node, v);
+      };
+#endif
+      if (full)
+        p2g_nodes<true>(n0, n1, n2, dx, o0, o1, o2, st, q, put);
+      else
+        p2g_nodes<false>(n0, n1, n2, dx, o0, o1, o2, st, q, put);
+    }
+  }
+  __syncthreads();
+
+  // ---- flush: per tile, the 2x2x2 node blocks it overlaps, 64 lanes = one
+  // block's 64 nodes (coalesced), one wave-uniform (block, field) per step ----
+  const size_t nodes = G.nodes;
+  for (int e = t; e < ntiles * 8 * kTileFields * 64; e += 256) {
+    const int l = e & 63;
+    const int r = e >> 6;
+    const int f = r % kTileFields;
+    const int r2 = r / kTileFields;
+    const int blk = r2 & 7;
+    const int ti = r2 >> 3;
+    const int lx = (blk & 1) * 4 + (l & 3), ly = ((blk >> 1) & 1) * 4 + ((l >> 2) & 3),
+              lz = (blk >> 2) * 4 + (l >> 4);
+    if (lx >= kTileSide || ly >= kTileSide || lz >= kTileSide) continue;
+    const long long v = tile[ti * kTileVals + f * kTileNodes + (lz * kTileSide + ly) * kTileSide + lx];
     if (v == 0) continue;
-    const int ti = e / kTileVals;
-    const int r = e - ti * kTileVals;
-    const int f = r / kTileNodes;
-    const int nn = r - f * kTileNodes;
-    const int lz = nn / 36, ly = (nn / 6) % 6, lx = nn % 6;
-    const SceneDev& S = sc[tsc[ti]];
+    const SceneDev& T = sc[tsc[ti]];
     const int gx = torg[ti][0] + lx, gy = torg[ti][1] + ly, gz = torg[ti][2] + lz;
-    if (gx >= S.n[0] || gy >= S.n[1] || gz >= S.n[2]) continue;
-    acc_global(G, f, node_addr(S, gx, gy, gz), v);
+    if (gx >= T.n[0] || gy >= T.n[1] || gz >= T.n[2]) continue;
+    const uint32_t node = node_addr(T, gx, gy, gz);
+#if AEMH_DET
+    atomicAdd((unsigned long long*)(G.accq + (size_t)f * nodes + node), (unsigned long long)v);
+#else
+    const int g = f == 0 ? 0 : (f < 4 ? 1 : 2);
+    const float fv = (float)((double)v * (double)__int_as_float((127 - 54 + eg[g]) << 23));
+    unsafeAtomicAdd(G.accf + (size_t)f * nodes + node, fv);
+#endif
   }
 }
 
@@ -213,10 +363,10 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
   const uint32_t lane = __lane_id();
   const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * 256u) >> 6;
-  const uint32_t nact = s.counters[0];
+  const uint32_t nact = s.counters[kCntActive];
   for (uint32_t a = wave; a < nact; a += nwaves) {
     const uint32_t b = s.active[a];
-    const SceneDev& S = sc[scene_of_block(sc, ns, b)];
+    const SceneDev S = sc[scene_of_block(sc, ns, b)];
     const uint32_t lb = b - S.block_off;
     const int bx = (int)(lb % (uint32_t)S.nb[0]);
     const int by = (int)((lb / (uint32_t)S.nb[0]) % (uint32_t)S.nb[1]);
@@ -233,6 +383,12 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
       acc[f] = G.accf[(size_t)f * G.nodes + node];
 #endif
     }
+#if AEMH_DET
+    // sum |c| >= 2^63 raw: some order of the int64 adds could have wrapped
+    // (and the saturating FixedPoint sum would depend on order).
+    if ((unsigned long long)G.accq[(size_t)7 * G.nodes + node] >= (1ull << 31))
+      s.counters[kCntOverflow] = 1u;
+#endif
     const float m = acc[0];
     float v0 = acc[1], v1 = acc[2], v2 = acc[3];
     if (m > 1e-12f) {
@@ -262,6 +418,15 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
   }
 }
 
+// Node (v, m) of the block-major grid; callers clamp coords into the grid so
+// every load is in bounds.
+__device__ __forceinline__ float4 load_node(const float4* __restrict__ gv, uint32_t boff, int nb0,
+                                            int nb1, int gx, int gy, int gz) {
+  const uint32_t b = boff + (uint32_t)(((gz >> 2) * nb1 + (gy >> 2)) * nb0 + (gx >> 2));
+  const uint32_t node = (b << 6) | (uint32_t)(((gz & 3) << 4) | ((gy & 3) << 2) | (gx & 3));
+  return gv[node];
+}
+
 // G2P: sorted position j reads working slot perm[j] of `cur` and writes slot j
 // of `nxt`, so the particle arrays come out ordered by base block.
 template <int STORE>
@@ -271,15 +436,27 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
   const uint32_t j = blockIdx.x * 256u + threadIdx.x;
   if (j >= n) return;
   const uint32_t p = s.perm[j];
-  const SceneDev& S = sc[scene_of(sc, ns, j)];
+  const SceneDev S = sc[scene_of(sc, ns, j)];
+  const float4* __restrict__ gvp = G.gv;
   const float px = cur.x[0][p], py = cur.x[1][p], pz = cur.x[2][p];
   const Stencil st = make_stencil(S, px, py, pz);
+  const int n0 = S.n[0], n1 = S.n[1], n2 = S.n[2];
+  const int nb0 = S.nb[0], nb1 = S.nb[1];
+  const uint32_t boff = S.block_off;
   float wx[3], wy[3], wz[3];
+  int cx[3], cy[3], cz[3];
+  bool ix[3], iy[3], iz[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     wx[a] = b_spline((float)a - st.fx[0]);
     wy[a] = b_spline((float)a - st.fx[1]);
     wz[a] = b_spline((float)a - st.fx[2]);
+    ix[a] = st.g[0] + a < n0;
+    iy[a] = st.g[1] + a < n1;
+    iz[a] = st.g[2] + a < n2;
+    cx[a] = min(st.g[0] + a, n0 - 1);
+    cy[a] = min(st.g[1] + a, n1 - 1);
+    cz[a] = min(st.g[2] + a, n2 - 1);
   }
   const float dt = S.dt;
   const uint16_t mi = cur.mat[p];
@@ -295,15 +472,13 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
       for (int dj = 0; dj < 3; dj++)
 #pragma unroll
         for (int di = 0; di < 3; di++) {
-          const float w = wx[di] * (wy[dj] * wz[dk]);
-          if (w < 1e-10f) continue;
-          const int gx = st.g[0] + di, gy = st.g[1] + dj, gz = st.g[2] + dk;
-          if (gx >= S.n[0] || gy >= S.n[1] || gz >= S.n[2]) continue;
-          const float4 g = G.gv[node_addr(S, gx, gy, gz)];
-          if (g.w < 1e-12f) continue;
-          nv0 += w * g.x;
-          nv1 += w * g.y;
-          nv2 += w * g.z;
+          const float w0 = wx[di] * (wy[dj] * wz[dk]);
+          const float4 g = load_node(gvp, boff, nb0, nb1, cx[di], cy[dj], cz[dk]);
+          const bool keep = !(w0 < 1e-10f) && ix[di] && iy[dj] && iz[dk] && !(g.w < 1e-12f);
+          const float w = keep ? w0 : 0.0f;
+          nv0 += w * (keep ? g.x : 0.0f);
+          nv1 += w * (keep ? g.y : 0.0f);
+          nv2 += w * (keep ? g.z : 0.0f);
         }
     if (!isfinite(nv0) || !isfinite(nv1) || !isfinite(nv2)) nv0 = nv1 = nv2 = 0.0f;
     nv0 = fminf(fmaxf(nv0, -50.0f), 50.0f);
@@ -360,6 +535,7 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
     dya[a] = ((float)(st.g[1] + a) + 0.5f) * S.dx + S.o[1] - py;
     dza[a] = ((float)(st.g[2] + a) + 0.5f) * S.dx + S.o[2] - pz;
   }
+  const float k4 = S.four_inv_dx2;
   float nv[3] = {0.0f, 0.0f, 0.0f};
   float nc[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   float gv[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
@@ -369,19 +545,17 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
     for (int dj = 0; dj < 3; dj++)
 #pragma unroll
       for (int di = 0; di < 3; di++) {
-        const float w = wx[di] * wy[dj] * wz[dk];
-        if (w < 1e-10f) continue;
-        const int gx = st.g[0] + di, gy = st.g[1] + dj, gz = st.g[2] + dk;
-        if (gx >= S.n[0] || gy >= S.n[1] || gz >= S.n[2]) continue;
-        const float4 g = G.gv[node_addr(S, gx, gy, gz)];
-        if (g.w < 1e-12f) continue;
-        nv[0] += w * g.x;
-        nv[1] += w * g.y;
-        nv[2] += w * g.z;
-        const float scale = S.four_inv_dx2 * w;
+        const float w0 = wx[di] * wy[dj] * wz[dk];
+        const float4 g = load_node(gvp, boff, nb0, nb1, cx[di], cy[dj], cz[dk]);
+        const bool keep = !(w0 < 1e-10f) && ix[di] && iy[dj] && iz[dk] && !(g.w < 1e-12f);
+        const float w = keep ? w0 : 0.0f;
+        const float gg[3] = {keep ? g.x : 0.0f, keep ? g.y : 0.0f, keep ? g.z : 0.0f};
+        nv[0] += w * gg[0];
+        nv[1] += w * gg[1];
+        nv[2] += w * gg[2];
+        const float scale = k4 * w;
         const float d[3] = {dxa[di], dya[dj], dza[dk]};
         const float dw[3] = {dwx[di], dwy[dj], dwz[dk]};
-        const float gg[3] = {g.x, g.y, g.z};
 #pragma unroll
         for (int r = 0; r < 3; r++)
 #pragma unroll

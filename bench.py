@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""bench.py -- MI355X MLS-MPM step throughput (particle-updates/s).
+
+Workload of `value`: 1,048,576 snow particles (dam-break block
+[0,0.4]x[0,0.6]x[0,1]) on a 128^3 grid, dx 1/128, dt 5e-5, FP32 mode, one
+substep per step, inputs resident in HBM.  One step = sort by block key,
+P2G, grid update, G2P.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
+
+Prints one JSON line.  The timed region launches every kernel eagerly with
+HIP events around each launch on the context stream, so per-kernel durations
+(and the roofline of the dominant kernel) come from the same K steps; each
+aemh_step call returns after a stream synchronize, which brackets the region.
+`extra` adds the 16M-particle sparse-grid workload, the deterministic Q32.32
+mode, graph-replayed timings and the batched-scenes speedup.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ae-engine_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-updates/s at 1M & 16M particles; %HBM peak; batched vs plain speedup"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes (counted once per step, FP32 storage): per particle
+#   P2G reads x 12, v 12, C 36, F 36, J 4, J_p 4, mass 4, material 2 = 110 B
+#   G2P reads x 12, F 36, J_p 4 and writes x, v, C, F, J, J_p = 104 B -> 156 B
+#   whole step 206 B + 8 B (J_p read+write)                         = 214 B
+# per active grid node: P2G writes 28 B (m, p, f); update reads 28, writes
+#   16 (v, m); G2P reads 16 -> step 88 B; P2G 28 B; G2P 16 B.
+BYTES = {"p2g": (110, 28), "g2p": (156, 16), "grid_update": (0, 44), "step": (214, 88)}
+KERNELS = ["step_begin", "count", "scan", "scatter", "clear", "p2g", "grid_update", "g2p"]
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed_steps(eng, dt, k):
+    t0 = time.perf_counter()
+    for _ in range(k):
+        eng.step(dt)
+    return time.perf_counter() - t0
+
+
+def run_main(W, aemh, steps, warmup, accum=None):
+    wl = W.c2_snow_dam()
+    if accum is not None:
+        wl.accum = accum
+    eng = aemh.Engine()
+    W.load_engine(eng, wl)
+    for _ in range(warmup):
+        eng.step(wl.dt)
+    eng.profile(True)
+    eng.profile_reset()
+    el = timed_steps(eng, wl.dt, steps)
+    kern = {k: eng.profile_get(k)[0] for k in KERNELS}
+    eng.profile(False)
+    blocks, npart = eng.last_substep_info()
+    el_graph = timed_steps(eng, wl.dt, steps)
+    eng.close()
+    return wl, el, el_graph, kern, blocks, npart
+
+
+def roofline_of(kern, name, npart, blocks):
+    per_p, per_n = BYTES[name]
+    nbytes = npart * per_p + blocks * 64 * per_n
+    ms = kern[name]
+    achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "algorithmic_bytes": int(nbytes), "avg_ms": round(ms, 4)}
+
+
+def cpu_baseline(W, aemh):
+    from oracle import refcpu as R
+    wl = W.c2_snow_dam()
+    p, m, e = aemh.material_arrays(wl.materials)
+    M = R.Materials(p, m, e)
+    nx, ny, nz, dx, o = wl.grid
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    cfg = R.make_config(nx, ny, nz, dx, o, 1, wl.gravity, 0.995, wl.bc_margin, 0.0, R.ACCUM_F32,
+                        R.CLAMP_AUTO, threads=cores)
+    st = R.State(wl.pos, mass=wl.mass, mat=wl.mat)
+    R.step(cfg, M, st, wl.dt)  # warm-up
+    n_steps = 3
+    t0 = time.perf_counter()
+    for _ in range(n_steps):
+        R.step(cfg, M, st, wl.dt)
+    el = time.perf_counter() - t0
+    return {"value": wl.n * n_steps / el, "unit": "particle-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"oracle (C++ restatement, per-thread grid fold + merge like the "
+                      f"reference's rayon path) on the same 1,048,576-particle 128^3 snow workload, "
+                      f"{n_steps} steps after 1 warm-up, {cores} threads"}
+
+
+def extra_c3(W, aemh, steps=10):
+    wl = W.c3_three_boxes()
+    eng = aemh.Engine()
+    W.load_engine(eng, wl)
+    for _ in range(3):
+        eng.step(wl.dt)
+    el = timed_steps(eng, wl.dt, steps)
+    eng.profile(True)
+    eng.profile_reset()
+    timed_steps(eng, wl.dt, 3)
+    kern = {k: eng.profile_get(k)[0] for k in KERNELS}
+    blocks, npart = eng.last_substep_info()
+    eng.close()
+    total_blocks = 64 ** 3
+    step_bytes = npart * BYTES["step"][0] + blocks * 64 * BYTES["step"][1]
+    return {
+        "workload": "c3_three_boxes_16m_256: 16,777,216 particles (fluid / jelly / snow boxes), "
+                    "256^3 grid of 4^3 blocks, dt 5e-5, FP32",
+        "particle_updates_per_s": wl.n * steps / el,
+        "ms_per_step": el / steps * 1e3,
+        "active_block_fraction": blocks / total_blocks,
+        "step_hbm_frac": round(step_bytes / (el / steps) / 1e9 / HBM_PEAK_GBS, 4),
+        "g2p": roofline_of(kern, "g2p", npart, blocks),
+        "p2g": roofline_of(kern, "p2g", npart, blocks),
+        "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
+    }
+
+
+def extra_batched(W, aemh, steps=5):
+    wl = W.c4_batched_scenes()
+    eng = aemh.Engine()
+    W.load_engine(eng, wl)
+    eng.batch_step(2)
+    t0 = time.perf_counter()
+    eng.batch_step(steps)
+    el_b = time.perf_counter() - t0
+    eng.close()
+    # plain: one context per scene, one aemh_step call per scene and step
+    engines = []
+    for sc in wl.scenes:
+        sl = slice(sc.first, sc.first + sc.count)
+        e = aemh.Engine()
+        cfg = wl.config()
+        e.set_config(cfg)
+        e.set_grid(sc.nx, sc.ny, sc.nz, sc.dx, tuple(sc.origin))
+        e.set_materials(wl.materials)
+        e.upload(wl.pos[sl], mass=wl.mass[sl], mat=wl.mat[sl])
+        e.step(sc.dt)
+        engines.append((e, sc.dt))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for e, dt in engines:
+            e.step(dt)
+    el_p = time.perf_counter() - t0
+    for e, _ in engines:
+        e.close()
+    return {"workload": f"c4_batched_256_scenes: {len(wl.scenes)} scenes, {wl.n} particles, "
+                        f"grids 32^3..128^3, NH + snow",
+            "batched_particle_updates_per_s": wl.n * steps / el_b,
+            "plain_particle_updates_per_s": wl.n * steps / el_p,
+            "batched_vs_plain_speedup": el_p / el_b}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: every box here has one GPU and the MPM step does "
+              "not shard across GPUs in this build (see DESIGN.md for how it would).",
+              file=sys.stderr)
+        sys.exit(2)
+    import aemh
+    from aemh import workloads as W
+
+    log("[bench] main workload: C2 1M snow particles, 128^3")
+    wl, el, el_graph, kern, blocks, npart = run_main(W, aemh, args.steps, args.warmup)
+    value = wl.n * args.steps / el
+    dominant = max(("p2g", "g2p", "grid_update"), key=lambda k: kern[k])
+    step_bytes = npart * BYTES["step"][0] + blocks * 64 * BYTES["step"][1]
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded xoshiro256** particles (seed 42+index), v=0, F=I",
+        "config": {"workload": "c2_snow_1m_128: 1,048,576 snow particles in a dam-break block, "
+                               "128^3 grid, dx 1/128, dt 5e-5, FP32 P2G",
+                   "particles": wl.n, "grid": [128, 128, 128], "accum": "f32", "substeps": 1},
+        "roofline": roofline_of(kern, dominant, npart, blocks),
+        "step_hbm_frac": round(step_bytes / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
+    }
+    extra = {"graph_replay_particle_updates_per_s": wl.n * args.steps / el_graph,
+             "g2p_roofline": roofline_of(kern, "g2p", npart, blocks),
+             "active_blocks": blocks}
+    if not args.no_extras:
+        log("[bench] deterministic Q32.32 mode")
+        wq, elq, elq_g, kq, _, _ = run_main(W, aemh, max(5, args.steps // 5), 2,
+                                              accum=aemh.ACCUM_Q32_32)
+        extra["q32_particle_updates_per_s"] = wq.n * max(5, args.steps // 5) / elq
+        log("[bench] C3 16M particles")
+        extra["c3"] = extra_c3(W, aemh)
+        log("[bench] C4 batched vs plain")
+        extra["c4"] = extra_batched(W, aemh)
+    line["extra"] = extra
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle)")
+        line["cpu_baseline"] = cpu_baseline(W, aemh)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

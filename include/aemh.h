@@ -161,6 +161,11 @@ aemh_status aemh_download_grid_q32(aemh_ctx* ctx, uint32_t scene, int64_t* q7);
  * energy (ae_physics/src/mpm.rs:752-771 diagnostics, in FP64). out[5]. */
 aemh_status aemh_conservation_f64(aemh_ctx* ctx, double* out5);
 
+/* Work of the last substep: active 4^3 node blocks (grid nodes touched =
+ * 64 x blocks) and particles advanced. */
+aemh_status aemh_last_substep_info(aemh_ctx* ctx, uint32_t* active_blocks,
+                                   uint32_t* particles);
+
 /* Per-kernel HIP-event timing on the context stream (eager launches). */
 aemh_status aemh_profile_enable(aemh_ctx* ctx, int on);
 aemh_status aemh_profile_get(aemh_ctx* ctx, const char* kernel, double* avg_ms,

@@ -1059,6 +1059,15 @@ int ref_svd3_d(const double* F, double* U, double* sig, double* V) {
   return 1;
 }
 
+void ref_snow_clamp(const float* ext3, float* F, float* Jp) {
+  Mat<float> m{};
+  m.theta_c = ext3[0];
+  m.theta_s = ext3[1];
+  m.xi = ext3[2];
+  m.model = REF_SNOW;
+  snow_clamp(m, F, Jp);
+}
+
 int ref_step_f32(const RefConfig* cfg, const RefMaterials* mats, uint32_t n, float* pos,
                  float* vel, float* F, float* J, float* C, const float* mass,
                  const uint16_t* mat, float* Jp, const uint32_t* idx, uint32_t n_idx,

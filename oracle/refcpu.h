@@ -82,6 +82,8 @@ int ref_compute_stress(int model, const float* params7, const float* ext3,
 // F = U diag(sig) V^T, U and V rotations.  Returns 1 if FP64 was used.
 int ref_svd3(const float* F, float* U, float* sig, float* V);
 int ref_svd3_d(const double* F, double* U, double* sig, double* V);
+// Snow clamp of G2P (no reference counterpart): F and Jp updated in place.
+void ref_snow_clamp(const float* ext3, float* F, float* Jp);
 
 // ---- one full step (substeps inside) -----------------------------------
 // Arrays are the reference's MpssBuffer layout: pos/vel [n][3], F/C [n][9]

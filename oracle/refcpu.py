@@ -73,6 +73,8 @@ def lib():
         L.ref_svd3.argtypes = [fp, fp, fp, fp]
         L.ref_svd3_d.restype = C.c_int
         L.ref_svd3_d.argtypes = [dp, dp, dp, dp]
+        L.ref_snow_clamp.restype = None
+        L.ref_snow_clamp.argtypes = [fp, fp, fp]
         L.ref_step_f32.restype = C.c_int
         L.ref_step_f32.argtypes = [C.POINTER(RefConfig), C.POINTER(RefMaterials), C.c_uint32,
                                    fp, fp, fp, fp, fp, fp, C.POINTER(C.c_uint16), fp,
@@ -218,6 +220,14 @@ def svd3(F):
     V = np.zeros(9, np.float32)
     used64 = lib().ref_svd3(_fp(Fa), _fp(U), _fp(s), _fp(V))
     return U.reshape(3, 3), s, V.reshape(3, 3), bool(used64)
+
+
+def snow_clamp(F, Jp, theta_c=2.5e-2, theta_s=7.5e-3, xi=10.0):
+    e = np.array([theta_c, theta_s, xi], np.float32)
+    Fa = np.ascontiguousarray(np.asarray(F, np.float32).reshape(9)).copy()
+    j = np.array([Jp], np.float32)
+    lib().ref_snow_clamp(_fp(e), _fp(Fa), _fp(j))
+    return Fa.reshape(3, 3), float(j[0])
 
 
 def svd3_d(F):
