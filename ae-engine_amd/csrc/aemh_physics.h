@@ -210,23 +210,39 @@ __device__ __forceinline__ void compute_stress(const MatDev& M, const float* F, 
   }
 }
 
-// One-sided Jacobi SVD (Hestenes), 6 sweeps over (0,1),(0,2),(1,2); U and V
-// rotations, a reflection carried by the smallest singular value.  Same
+// One-sided Jacobi SVD (Hestenes): at most 6 sweeps over (0,1),(0,2),(1,2);
+// U and V rotations, a reflection carried by the smallest singular value.
+// A rotation is skipped when gamma^2 <= eps^2 alpha beta (columns orthogonal
+// to working precision, eps = 2^-24 / 2^-53); a sweep that skips all three
+// leaves every value unchanged, so stopping there equals running on.  Same
 // operation sequence as the oracle's svd3_jacobi.
 template <typename T>
-__device__ __noinline__ void svd3_jacobi(const T* F, T* U, T* sig, T* V) {
+struct JacobiTol;
+template <>
+struct JacobiTol<float> {
+  static constexpr float v = 3.5527136788005009e-15f;  // 2^-48
+};
+template <>
+struct JacobiTol<double> {
+  static constexpr double v = 1.2325951644078310e-32;  // 2^-106
+};
+
+template <typename T>
+__device__ __forceinline__ void svd3_jacobi_body(const T* F, T* U, T* sig, T* V) {
   T a[9];
 #pragma unroll
   for (int i = 0; i < 9; i++) a[i] = F[i];
   identity3(V);
   for (int sweep = 0; sweep < 6; sweep++) {
+    bool rotated = false;
 #pragma unroll
     for (int pq = 0; pq < 3; pq++) {
       const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
       T alpha = a[p] * a[p] + a[3 + p] * a[3 + p] + a[6 + p] * a[6 + p];
       T beta = a[q] * a[q] + a[3 + q] * a[3 + q] + a[6 + q] * a[6 + q];
       T gamma = a[p] * a[q] + a[3 + p] * a[3 + q] + a[6 + p] * a[6 + q];
-      if (gamma == T(0)) continue;
+      if (gamma * gamma <= JacobiTol<T>::v * (alpha * beta)) continue;
+      rotated = true;
       T zeta = (beta - alpha) / (T(2.0) * gamma);
       T t = T(1.0) / (fabs(zeta) + sqrt(T(1.0) + zeta * zeta));
       if (zeta < T(0)) t = -t;
@@ -242,6 +258,7 @@ __device__ __noinline__ void svd3_jacobi(const T* F, T* U, T* sig, T* V) {
         V[3 * r + q] = s * vp + c * vq;
       }
     }
+    if (!rotated) break;
   }
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -283,15 +300,21 @@ __device__ __noinline__ void svd3_jacobi(const T* F, T* U, T* sig, T* V) {
   }
 }
 
+// The FP64 solve is rare (cond > 1e4): kept out of line so its registers do
+// not weigh on the caller.
+__device__ __noinline__ void svd3_jacobi_f64(const double* F, double* U, double* sig, double* V) {
+  svd3_jacobi_body<double>(F, U, sig, V);
+}
+
 // FP32 SVD with the FP64 fallback for cond(F) > 1e4.
 __device__ __forceinline__ void svd3(const float* F, float* U, float* sig, float* V) {
-  svd3_jacobi<float>(F, U, sig, V);
+  svd3_jacobi_body<float>(F, U, sig, V);
   float smax = fmaxf(fabsf(sig[0]), fmaxf(fabsf(sig[1]), fabsf(sig[2])));
   float smin = fminf(fabsf(sig[0]), fminf(fabsf(sig[1]), fabsf(sig[2])));
   if (!(smin * 1e4f >= smax)) {
     double Fd[9], Ud[9], sd[3], Vd[9];
     for (int i = 0; i < 9; i++) Fd[i] = F[i];
-    svd3_jacobi<double>(Fd, Ud, sd, Vd);
+    svd3_jacobi_f64(Fd, Ud, sd, Vd);
     for (int i = 0; i < 9; i++) {
       U[i] = (float)Ud[i];
       V[i] = (float)Vd[i];
@@ -300,20 +323,40 @@ __device__ __forceinline__ void svd3(const float* F, float* U, float* sig, float
   }
 }
 
-// Snow plasticity after the F update (Stomakhin et al. 2013): clamp the
-// singular values to [1 - theta_c, 1 + theta_s], J_p *= sigma / sigma_hat,
-// F = U diag(sigma_hat) V^T.  No reference counterpart in the FP32 solver.
-__device__ __noinline__ void snow_clamp(const MatDev& M, float* F, float* Jp) {
+// Snow plasticity after the F update (Stomakhin et al. 2013): singular values
+// outside [1 - theta_c, 1 + theta_s] are clamped, J_p *= sigma / sigma_hat,
+// F = U diag(sigma_hat) V^T.  F and J_p are left unchanged when nothing is
+// clamped, which the Gershgorin discs of F^T F (eigenvalues sigma^2) decide
+// without an SVD when they lie inside [lo^2, hi^2].  No reference counterpart
+// in the FP32 solver.
+__device__ __forceinline__ void snow_clamp(const MatDev& M, float* F, float* Jp) {
+  const float lo = 1.0f - M.theta_c, hi = 1.0f + M.theta_s;
+  {
+    const float c00 = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
+    const float c11 = F[1] * F[1] + F[4] * F[4] + F[7] * F[7];
+    const float c22 = F[2] * F[2] + F[5] * F[5] + F[8] * F[8];
+    const float c01 = fabsf(F[0] * F[1] + F[3] * F[4] + F[6] * F[7]);
+    const float c02 = fabsf(F[0] * F[2] + F[3] * F[5] + F[6] * F[8]);
+    const float c12 = fabsf(F[1] * F[2] + F[4] * F[5] + F[7] * F[8]);
+    const float lo2 = lo * lo, hi2 = hi * hi;
+    const float r0 = c01 + c02, r1 = c01 + c12, r2 = c02 + c12;
+    if (c00 - r0 >= lo2 && c00 + r0 <= hi2 && c11 - r1 >= lo2 && c11 + r1 <= hi2 &&
+        c22 - r2 >= lo2 && c22 + r2 <= hi2)
+      return;
+  }
   float U[9], s[3], V[9];
   svd3(F, U, s, V);
-  const float lo = 1.0f - M.theta_c, hi = 1.0f + M.theta_s;
   float sh[3];
-  float jp = *Jp;
+  bool clamped = false;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     sh[d] = fminf(fmaxf(s[d], lo), hi);
-    jp = jp * s[d] / sh[d];
+    clamped = clamped || sh[d] != s[d];
   }
+  if (!clamped) return;
+  float jp = *Jp;
+#pragma unroll
+  for (int d = 0; d < 3; d++) jp = jp * s[d] / sh[d];
   *Jp = jp;
 #pragma unroll
   for (int r = 0; r < 3; r++)

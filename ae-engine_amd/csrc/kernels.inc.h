@@ -539,7 +539,9 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
   float nv[3] = {0.0f, 0.0f, 0.0f};
   float nc[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   float gv[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
+  // One z-layer (9 node loads) in flight: ~133 VGPRs, 3 waves per SIMD.  The
+  // fully unrolled 27-node gather needs 256 VGPRs and runs 1.6x slower.
+#pragma unroll 1
   for (int dk = 0; dk < 3; dk++)
 #pragma unroll
     for (int dj = 0; dj < 3; dj++)
@@ -606,6 +608,16 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
     np[2] = hiz;
     nv[2] = fminf(nv[2], 0.0f) * 0.5f;
   }
+  // x, v and C are final here: store them before the F update so they are
+  // not live across the snow SVD.
+  nxt.x[0][j] = np[0];
+  nxt.x[1][j] = np[1];
+  nxt.x[2][j] = np[2];
+  nxt.v[0][j] = nv[0] * S.damping;
+  nxt.v[1][j] = nv[1] * S.damping;
+  nxt.v[2][j] = nv[2] * S.damping;
+#pragma unroll
+  for (int k = 0; k < 9; k++) put_c(nxt, j, k, nc[k], STORE);
   // Q9: F' = (I + dt grad v) F; validity; J = clamp(det F', 0.1, 10).
   float F[9], nf[9];
 #pragma unroll
@@ -639,17 +651,8 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
     identity3(nf);
     fj = 1.0f;
   }
-  nxt.x[0][j] = np[0];
-  nxt.x[1][j] = np[1];
-  nxt.x[2][j] = np[2];
-  nxt.v[0][j] = nv[0] * S.damping;
-  nxt.v[1][j] = nv[1] * S.damping;
-  nxt.v[2][j] = nv[2] * S.damping;
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
-    put_c(nxt, j, k, nc[k], STORE);
-    put_f(nxt, j, k, nf[k], STORE);
-  }
+  for (int k = 0; k < 9; k++) put_f(nxt, j, k, nf[k], STORE);
   nxt.J[j] = fj;
   nxt.Jp[j] = jp;
 }

@@ -465,12 +465,22 @@ int compute_stress(const Mat<R>& m, const R* F, R J, R Jp, int det, R* stress, R
 }
 
 // ------------------------------------------------------------------------
-// One-sided (Hestenes) Jacobi SVD, 6 fixed sweeps of (0,1),(0,2),(1,2).
+// One-sided (Hestenes) Jacobi SVD, at most 6 sweeps of (0,1),(0,2),(1,2).
+// A rotation is skipped when gamma^2 <= eps^2 alpha beta (eps = 2^-24 for
+// float, 2^-53 for double); a sweep with no rotation ends the loop, which
+// gives the same values as any further sweeps.
 // No reference counterpart: the reference's snow SVD is fixed-point Jacobi on
 // F^T F (ae_physics/src/mpm.rs:794-951), which squares cond(F).  This form
 // orthogonalises the columns of F directly; U and V are rotations and a
 // reflection is carried by negating the smallest singular value.
 // ------------------------------------------------------------------------
+template <class R>
+constexpr R jacobi_tol();
+template <>
+constexpr float jacobi_tol<float>() { return 3.5527136788005009e-15f; }  // 2^-48
+template <>
+constexpr double jacobi_tol<double>() { return 1.2325951644078310e-32; }  // 2^-106
+
 template <class R>
 void svd3_jacobi(const R* F, R* U, R* sig, R* V) {
   R a[9];
@@ -478,12 +488,14 @@ void svd3_jacobi(const R* F, R* U, R* sig, R* V) {
   identity3(V);
   const int P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
   for (int sweep = 0; sweep < 6; sweep++) {
+    bool rotated = false;
     for (int pq = 0; pq < 3; pq++) {
       int p = P[pq], q = Q[pq];
       R alpha = a[p] * a[p] + a[3 + p] * a[3 + p] + a[6 + p] * a[6 + p];
       R beta = a[q] * a[q] + a[3 + q] * a[3 + q] + a[6 + q] * a[6 + q];
       R gamma = a[p] * a[q] + a[3 + p] * a[3 + q] + a[6 + p] * a[6 + q];
-      if (gamma == R(0)) continue;
+      if (gamma * gamma <= jacobi_tol<R>() * (alpha * beta)) continue;
+      rotated = true;
       R zeta = (beta - alpha) / (R(2.0) * gamma);
       R t = R(1.0) / (fabs(zeta) + sqrt(R(1.0) + zeta * zeta));
       if (zeta < R(0)) t = -t;
@@ -498,6 +510,7 @@ void svd3_jacobi(const R* F, R* U, R* sig, R* V) {
         V[3 * r + q] = s * vp + c * vq;
       }
     }
+    if (!rotated) break;
   }
   for (int c = 0; c < 3; c++) {
     R n2 = a[c] * a[c] + a[3 + c] * a[3 + c] + a[6 + c] * a[6 + c];
@@ -565,18 +578,37 @@ int svd3_r(const double* F, double* U, double* sig, double* V) {
 
 // Snow plasticity clamp applied in G2P after F' = (I + dt grad v) F.
 // Stomakhin et al. 2013: sigma clamped to [1-theta_c, 1+theta_s];
-// J_p *= sigma/sigma_hat per axis; F <- U diag(sigma_hat) V^T.
+// J_p *= sigma/sigma_hat per axis; F <- U diag(sigma_hat) V^T.  F and J_p stay
+// unchanged when no singular value is clamped; the Gershgorin discs of F^T F
+// (whose eigenvalues are sigma^2) decide that without an SVD when they lie
+// inside [lo^2, hi^2].
 template <class R>
 void snow_clamp(const Mat<R>& m, R* F, R* Jp) {
+  R lo = R(1.0) - m.theta_c, hi = R(1.0) + m.theta_s;
+  {
+    R c00 = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
+    R c11 = F[1] * F[1] + F[4] * F[4] + F[7] * F[7];
+    R c22 = F[2] * F[2] + F[5] * F[5] + F[8] * F[8];
+    R c01 = fabs(F[0] * F[1] + F[3] * F[4] + F[6] * F[7]);
+    R c02 = fabs(F[0] * F[2] + F[3] * F[5] + F[6] * F[8]);
+    R c12 = fabs(F[1] * F[2] + F[4] * F[5] + F[7] * F[8]);
+    R lo2 = lo * lo, hi2 = hi * hi;
+    R r0 = c01 + c02, r1 = c01 + c12, r2 = c02 + c12;
+    if (c00 - r0 >= lo2 && c00 + r0 <= hi2 && c11 - r1 >= lo2 && c11 + r1 <= hi2 &&
+        c22 - r2 >= lo2 && c22 + r2 <= hi2)
+      return;
+  }
   R U[9], s[3], V[9];
   svd3_r(F, U, s, V);
-  R lo = R(1.0) - m.theta_c, hi = R(1.0) + m.theta_s;
   R sh[3];
-  R jp = *Jp;
+  bool clamped = false;
   for (int d = 0; d < 3; d++) {
     sh[d] = rmin(rmax(s[d], lo), hi);
-    jp = jp * s[d] / sh[d];
+    clamped = clamped || sh[d] != s[d];
   }
+  if (!clamped) return;
+  R jp = *Jp;
+  for (int d = 0; d < 3; d++) jp = jp * s[d] / sh[d];
   *Jp = jp;
   for (int r = 0; r < 3; r++)
     for (int c = 0; c < 3; c++)
