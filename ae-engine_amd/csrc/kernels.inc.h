@@ -24,14 +24,18 @@ namespace AEMH_NS {
 namespace {
 
 // Up to this many distinct base blocks of one workgroup's particles are
-// staged in LDS; the rare rest scatter straight to HBM.
-constexpr int kMaxTiles = 4;
+// staged in LDS; the rare rest scatter straight to HBM.  3 tiles (37 KB of
+// int64 FP32-mode accumulators) fit four workgroups per CU; 4 tiles fit three
+// and ran P2G 10-20% slower on the 1M and 16M workloads.
+constexpr int kMaxTiles = 3;
 #if AEMH_DET
 constexpr int kTileFields = kDetFields;  // 7 sums + the magnitude bound
 #else
 constexpr int kTileFields = 7;
 #endif
 constexpr int kTileVals = kTileFields * kTileNodes;
+// Lane layers of the P2G re-assignment (rank within a base cell, capped).
+constexpr int kLayers = 16;
 
 // Per-particle P2G inputs after the constitutive update.
 struct P2GPart {
@@ -114,35 +118,92 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
                                              uint32_t n, SortDev s, GridDev G) {
   __shared__ long long tile[kMaxTiles * kTileVals];
   __shared__ uint32_t skey[256];
-  __shared__ int tsc[kMaxTiles];
+  __shared__ uint32_t sperm[256];
+  __shared__ uint32_t ccnt[kMaxTiles * 64];
+  __shared__ uint32_t lcnt[kLayers];
+  __shared__ int16_t sslot[256];
+  __shared__ uint8_t order[256];
   __shared__ int torg[kMaxTiles][3];
+  __shared__ int tdim[kMaxTiles][5];  // scene grid n[0..2], nb[0..1] of the tile
+  __shared__ uint32_t tboff[kMaxTiles];
   __shared__ uint32_t wcnt[4];
   __shared__ int wexp[4][3];
 
   const int t = threadIdx.x;
   const int lane = __lane_id(), wave = t >> 6;
-  const uint32_t j = blockIdx.x * 256u + (uint32_t)t;
-  const bool valid = j < n;
-  uint32_t p = 0, key = 0xffffffffu;
-  if (valid) {
-    p = s.perm[j];
-    key = s.key[p];
-  }
-  skey[t] = key;
-  __syncthreads();
-  const bool head = valid && (t == 0 || skey[t - 1] != key);
-  const unsigned long long hb = __ballot(head);
-  const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-  if (lane == 0) wcnt[wave] = (uint32_t)__popcll(hb);
-  __syncthreads();
-  uint32_t pre = 0, tot = 0;
+  const uint32_t j0 = blockIdx.x * 256u;
+  int slot, ntiles;
+  {
+    // ---- tiles: one per distinct base block, in sorted order ----
+    const uint32_t j = j0 + (uint32_t)t;
+    const bool valid = j < n;
+    uint32_t p = 0, key = 0xffffffffu;
+    if (valid) {
+      p = s.perm[j];
+      key = s.key[p];
+    }
+    skey[t] = key;
+    sperm[t] = p;
+    if (t < kLayers) lcnt[t] = 0;
+    __syncthreads();
+    const bool head = valid && (t == 0 || skey[t - 1] != key);
+    const unsigned long long hb = __ballot(head);
+    const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    if (lane == 0) wcnt[wave] = (uint32_t)__popcll(hb);
+    for (int e = t; e < kMaxTiles * 64; e += 256) ccnt[e] = 0;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
 #pragma unroll
-  for (int w = 0; w < 4; w++) {
-    if (w < wave) pre += wcnt[w];
-    tot += wcnt[w];
+    for (int w = 0; w < 4; w++) {
+      if (w < wave) pre += wcnt[w];
+      tot += wcnt[w];
+    }
+    slot = (int)(pre + (uint32_t)__popcll(hb & le)) - 1;
+    ntiles = (int)min(tot, (uint32_t)kMaxTiles);
+    Stencil st = {{0, 0, 0}, {0.f, 0.f, 0.f}};
+    int sid = 0;
+    if (valid) {
+      sid = scene_of(sc, ns, j);
+      st = make_stencil(sc[sid], P.x[0][p], P.x[1][p], P.x[2][p]);
+    }
+    if (head && slot < kMaxTiles) {
+      const SceneDev& T = sc[sid];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        torg[slot][a] = (min(st.g[a], T.n[a] - 1) >> 2) * 4;
+        tdim[slot][a] = T.n[a];
+      }
+      tdim[slot][3] = T.nb[0];
+      tdim[slot][4] = T.nb[1];
+      tboff[slot] = T.block_off;
+    }
+    sslot[t] = (int16_t)min(slot, 32767);
+    __syncthreads();
+
+    // ---- layers: rank of the particle among the workgroup's particles of its
+    // base cell.  Lanes are re-assigned layer by layer, so the 64 lanes of one
+    // LDS atomic hit distinct nodes (the first kLayers - 1 layers). ----
+    int layer = kLayers - 1;
+    if (valid && slot < kMaxTiles) {
+      int c = 0;
+#pragma unroll
+      for (int a = 2; a >= 0; a--) c = c * 4 + min(max(st.g[a] - torg[slot][a], 0), 3);
+      layer = (int)min(atomicAdd(&ccnt[slot * 64 + c], 1u), (uint32_t)(kLayers - 1));
+    }
+    const uint32_t rl = atomicAdd(&lcnt[layer], 1u);
+    __syncthreads();
+    uint32_t off = 0;
+    for (int l = 0; l < layer; l++) off += lcnt[l];
+    order[off + rl] = (uint8_t)t;
+    for (int e = t; e < ntiles * kTileVals; e += 256) tile[e] = 0;
+    __syncthreads();
   }
-  const int slot = (int)(pre + (uint32_t)__popcll(hb & le)) - 1;
-  const int ntiles = (int)min(tot, (uint32_t)kMaxTiles);
+  // This lane now carries workgroup particle u (sorted position j0 + u).
+  const int u = order[t];
+  const uint32_t j = j0 + (uint32_t)u;
+  const bool valid = j < n;
+  const uint32_t p = sperm[u];
+  slot = sslot[u];
 
   int sid = 0;
   SceneDev S{};
@@ -152,12 +213,6 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
     S = sc[sid];
     st = make_stencil(S, P.x[0][p], P.x[1][p], P.x[2][p]);
   }
-  if (head && slot < kMaxTiles) {
-    tsc[slot] = sid;
-#pragma unroll
-    for (int a = 0; a < 3; a++) torg[slot][a] = (min(st.g[a], S.n[a] - 1) >> 2) * 4;
-  }
-  for (int e = t; e < ntiles * kTileVals; e += 256) tile[e] = 0;
 
   // ---- particle inputs and stress ----------------------------------------
   const bool full = !S.vel_only;
@@ -343,10 +398,10 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
     if (lx >= kTileSide || ly >= kTileSide || lz >= kTileSide) continue;
     const long long v = tile[ti * kTileVals + f * kTileNodes + (lz * kTileSide + ly) * kTileSide + lx];
     if (v == 0) continue;
-    const SceneDev& T = sc[tsc[ti]];
     const int gx = torg[ti][0] + lx, gy = torg[ti][1] + ly, gz = torg[ti][2] + lz;
-    if (gx >= T.n[0] || gy >= T.n[1] || gz >= T.n[2]) continue;
-    const uint32_t node = node_addr(T, gx, gy, gz);
+    if (gx >= tdim[ti][0] || gy >= tdim[ti][1] || gz >= tdim[ti][2]) continue;
+    const uint32_t b = tboff[ti] + (uint32_t)(((gz >> 2) * tdim[ti][4] + (gy >> 2)) * tdim[ti][3] + (gx >> 2));
+    const uint32_t node = (b << 6) | (uint32_t)(((gz & 3) << 4) | ((gy & 3) << 2) | (gx & 3));
 #if AEMH_DET
     atomicAdd((unsigned long long*)(G.accq + (size_t)f * nodes + node), (unsigned long long)v);
 #else
