@@ -71,6 +71,7 @@ struct aemh_ctx {
   std::vector<SceneDev> scd_h;
   SceneDev* scd_d = nullptr;
   size_t scd_cap = 0;
+  bool scd_valid = false;  // scd_d holds scd_h
   uint32_t nb_total = 0;
   // grid + sort
   GridDev grid{};
@@ -325,14 +326,21 @@ aemh_status set_scene_table(aemh_ctx* c, const std::vector<aemh_scene>& sc) {
   if (d.size() > c->scd_cap) {
     if (c->scd_d) hipFree(c->scd_d);
     c->scd_d = nullptr;
+    c->scd_valid = false;
     HIPTRY(c, hipMalloc(&c->scd_d, d.size() * sizeof(SceneDev)));
     c->scd_cap = d.size();
     invalidate_graphs(c);
   }
   if (shape_changed) invalidate_graphs(c);
+  // A repeated step with the same scene reuses the table already on device.
+  const bool same = !shape_changed && c->scd_valid &&
+                    memcmp(c->scd_h.data(), d.data(), d.size() * sizeof(SceneDev)) == 0;
   c->scd_h = d;
-  HIPTRY(c, hipMemcpyAsync(c->scd_d, d.data(), d.size() * sizeof(SceneDev), hipMemcpyHostToDevice,
-                           c->stream));
+  if (!same) {
+    HIPTRY(c, hipMemcpyAsync(c->scd_d, d.data(), d.size() * sizeof(SceneDev), hipMemcpyHostToDevice,
+                             c->stream));
+    c->scd_valid = true;
+  }
   return ensure_grid_sort(c);
 }
 

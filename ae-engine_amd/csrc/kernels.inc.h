@@ -94,13 +94,42 @@ __device__ __forceinline__ long long apply_sign(unsigned long long m, float x) {
   return x < 0.0f ? -(long long)m : (long long)m;
 }
 
+// round(y) as an int64 for |y| < 2^51, in four VALU: y + 1.5 2^52 lies in
+// [2^52, 2^53), where the double ulp is 1, so its bit pattern minus that of
+// 1.5 2^52 is the rounded integer in two's complement (the constant's low
+// word is zero, so only the high word is adjusted).
+__device__ __forceinline__ long long rint_i64(double y) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(y + 6755399441055744.0);
+  return (long long)(b - 0x4338000000000000ull);
+}
+
 #if AEMH_DET
-// FixedPoint::from_f32: trunc(v 2^32) with saturation, NaN -> 0.
+// FixedPoint::from_f32: trunc(v 2^32) with saturation, NaN -> 0.  The product
+// is exact in double; below 2^19 its truncation goes through rint_i64.
 __device__ __forceinline__ long long to_q32(float v) {
-  if (!(fabsf(v) < 2147483648.0f)) return v != v ? 0 : (v > 0.0f ? 0x7fffffffffffffffLL : (long long)0x8000000000000000ULL);
-  return apply_sign(trunc_abs_u64(fabsf(v) * 4294967296.0f), v);
+  const float av = fabsf(v);
+  if (av < 524288.0f) return rint_i64(trunc((double)v * 4294967296.0));
+  if (!(av < 2147483648.0f)) return v != v ? 0 : (v > 0.0f ? 0x7fffffffffffffffLL : (long long)0x8000000000000000ULL);
+  return apply_sign(trunc_abs_u64(av * 4294967296.0f), v);
 }
 #endif
+
+// Node t (< 216) of a 6^3 tile, enumerated block by block over the 2x2x2
+// 4^3-blocks the tile overlaps (64, 32, 32, 16, 32, 16, 16, 8 nodes), so the
+// lanes of one flush wave land in few 64-node grid blocks.
+__device__ __forceinline__ void tile_node_of(int t, int& lx, int& ly, int& lz) {
+  const int g = (t >= 64) + (t >= 96) + (t >= 128) + (t >= 144) + (t >= 176) + (t >= 192) + (t >= 208);
+  constexpr int kStart[8] = {0, 64, 96, 128, 144, 176, 192, 208};
+  int i = t;
+#pragma unroll
+  for (int k = 1; k < 8; k++)
+    if (g == k) i = t - kStart[k];
+  const int bx = g & 1, by = (g >> 1) & 1, bz = g >> 2;
+  const int wx = bx ? 2 : 4, wy = by ? 2 : 4;
+  lx = bx * 4 + i % wx;
+  ly = by * 4 + (i / wx) % wy;
+  lz = bz * 4 + i / (wx * wy);
+}
 
 // P2G.  One particle per thread over the block-sorted order.  The distinct
 // base blocks of the workgroup (usually one or two) each get a 6^3-node LDS
@@ -109,7 +138,7 @@ __device__ __forceinline__ long long to_q32(float v) {
 // the FP32 mode adds each value scaled by a per-workgroup power of two 2^s_g
 // (one per field group: mass, momentum, force) chosen so 256 contributions
 // cannot overflow, i.e. an exact block-floating-point sum with resolution
-// 2^-54 of the group's largest contribution.  Tiles are flushed block-
+// 2^-50 of the group's largest contribution.  Tiles are flushed block-
 // coalesced with one HBM atomic per non-zero node value.  The LDS and HBM
 // scatter paths are separate code so the LDS one stays ds_add.
 template <int STORE>
@@ -298,7 +327,7 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
   for (int g = 0; g < 3; g++) {
     eg[g] = max(max(wexp[0][g], wexp[1][g]), max(wexp[2][g], wexp[3][g]));
     eg[g] = max(eg[g], -70);
-    scale[g] = __int_as_float((127 + 54 - eg[g]) << 23);  // 2^(54 - e_g)
+    scale[g] = __int_as_float((127 + 50 - eg[g]) << 23);  // 2^(50 - e_g)
   }
 #endif
 
@@ -330,7 +359,7 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
 #else
       auto put = [&](int f, int di, int dj, int dk, int, int, int, float v) {
         const int g = f == 0 ? 0 : (f < 4 ? 1 : 2);
-        const long long a = apply_sign(trunc_abs_u64(fabsf(v) * scale[g]), v);
+        const long long a = rint_i64((double)(v * scale[g]));  // |v scale| < 2^50
         atomicAdd((unsigned long long*)&tile[tb + f * kTileNodes + (dk * kTileSide + dj) * kTileSide + di],
                   (unsigned long long)a);
       };
@@ -383,32 +412,31 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
   }
   __syncthreads();
 
-  // ---- flush: per tile, the 2x2x2 node blocks it overlaps, 64 lanes = one
-  // block's 64 nodes (coalesced), one wave-uniform (block, field) per step ----
-  const size_t nodes = G.nodes;
-  for (int e = t; e < ntiles * 8 * kTileFields * 64; e += 256) {
-    const int l = e & 63;
-    const int r = e >> 6;
-    const int f = r % kTileFields;
-    const int r2 = r / kTileFields;
-    const int blk = r2 & 7;
-    const int ti = r2 >> 3;
-    const int lx = (blk & 1) * 4 + (l & 3), ly = ((blk >> 1) & 1) * 4 + ((l >> 2) & 3),
-              lz = (blk >> 2) * 4 + (l >> 4);
-    if (lx >= kTileSide || ly >= kTileSide || lz >= kTileSide) continue;
-    const long long v = tile[ti * kTileVals + f * kTileNodes + (lz * kTileSide + ly) * kTileSide + lx];
-    if (v == 0) continue;
-    const int gx = torg[ti][0] + lx, gy = torg[ti][1] + ly, gz = torg[ti][2] + lz;
-    if (gx >= tdim[ti][0] || gy >= tdim[ti][1] || gz >= tdim[ti][2]) continue;
-    const uint32_t b = tboff[ti] + (uint32_t)(((gz >> 2) * tdim[ti][4] + (gy >> 2)) * tdim[ti][3] + (gx >> 2));
-    const uint32_t node = (b << 6) | (uint32_t)(((gz & 3) << 4) | ((gy & 3) << 2) | (gx & 3));
+  // ---- flush: thread t < 216 owns node t of every tile, one HBM atomic per
+  // non-zero value ----
+  if (t < kTileNodes) {
+    const size_t nodes = G.nodes;
+    int lx, ly, lz;
+    tile_node_of(t, lx, ly, lz);
+    const int li = (lz * kTileSide + ly) * kTileSide + lx;
+    for (int ti = 0; ti < ntiles; ti++) {
+      const int gx = torg[ti][0] + lx, gy = torg[ti][1] + ly, gz = torg[ti][2] + lz;
+      if (gx >= tdim[ti][0] || gy >= tdim[ti][1] || gz >= tdim[ti][2]) continue;
+      const uint32_t b = tboff[ti] + (uint32_t)(((gz >> 2) * tdim[ti][4] + (gy >> 2)) * tdim[ti][3] + (gx >> 2));
+      const uint32_t node = (b << 6) | (uint32_t)(((gz & 3) << 4) | ((gy & 3) << 2) | (gx & 3));
+#pragma unroll
+      for (int f = 0; f < kTileFields; f++) {
+        const long long v = tile[ti * kTileVals + f * kTileNodes + li];
+        if (v == 0) continue;
 #if AEMH_DET
-    atomicAdd((unsigned long long*)(G.accq + (size_t)f * nodes + node), (unsigned long long)v);
+        atomicAdd((unsigned long long*)(G.accq + (size_t)f * nodes + node), (unsigned long long)v);
 #else
-    const int g = f == 0 ? 0 : (f < 4 ? 1 : 2);
-    const float fv = (float)((double)v * (double)__int_as_float((127 - 54 + eg[g]) << 23));
-    unsafeAtomicAdd(G.accf + (size_t)f * nodes + node, fv);
+        const int g = f == 0 ? 0 : (f < 4 ? 1 : 2);
+        const float fv = (float)((double)v * (double)__int_as_float((127 - 50 + eg[g]) << 23));
+        unsafeAtomicAdd(G.accf + (size_t)f * nodes + node, fv);
 #endif
+      }
+    }
   }
 }
 
