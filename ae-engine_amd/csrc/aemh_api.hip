@@ -81,6 +81,7 @@ struct aemh_ctx {
   uint32_t sort_n_cap = 0, sort_nb_cap = 0;
   LaunchShape shape;
   bool tuned = false;
+  float tune_ms[3] = {0.0f, 0.0f, 0.0f};  // best P2G, grid update, G2P time
   // graphs
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   bool graph_valid = false;
@@ -382,6 +383,21 @@ void collect_profile(aemh_ctx* c) {
   c->pending.clear();
 }
 
+// Average time of `reps` back-to-back runs of fn on the context stream.
+template <typename Fn>
+aemh_status time_launches(aemh_ctx* c, int reps, float* ms, Fn&& fn) {
+  hipEvent_t a = get_event(c), b = get_event(c);
+  HIPTRY(c, hipEventRecord(a, c->stream));
+  for (int r = 0; r < reps; r++) fn();
+  HIPTRY(c, hipEventRecord(b, c->stream));
+  HIPTRY(c, hipEventSynchronize(b));
+  HIPTRY(c, hipEventElapsedTime(ms, a, b));
+  c->ev_pool.push_back(a);
+  c->ev_pool.push_back(b);
+  *ms /= (float)reps;
+  return AEMH_OK;
+}
+
 // Launch one substep on the context stream (eager or under capture).
 void enqueue_substep(aemh_ctx* c, int cur) {
   const PBuf a = c->work[cur].pb, b = c->work[cur ^ 1].pb;
@@ -488,9 +504,9 @@ aemh_status flush_work(aemh_ctx* c) {
   return AEMH_OK;
 }
 
-aemh_status single_scene(aemh_ctx* c, float total_dt, int vel_only) {
+// Scene table of a plain (single-grid) step.
+aemh_status prepare_single_scene(aemh_ctx* c, float sub_dt, int vel_only) {
   if (!c->grid_set) return fail(c, AEMH_ERR_STATE, "grid not set");
-  if (c->cfg.substeps == 0) return fail(c, AEMH_ERR_ARG, "substeps must be >= 1");
   aemh_status s = ensure_work(c);
   if (s != AEMH_OK) return s;
   aemh_scene sc{};
@@ -501,11 +517,17 @@ aemh_status single_scene(aemh_ctx* c, float total_dt, int vel_only) {
   sc.nz = c->nz;
   sc.dx = c->dx;
   for (int a = 0; a < 3; a++) sc.origin[a] = c->origin[a];
-  sc.dt = total_dt / (float)c->cfg.substeps;  // mpm_solver.rs:1075
+  sc.dt = sub_dt;
   sc.velocity_only = vel_only;
   sc.clamp_mode = c->cfg.clamp_mode;
   c->batch_mode = false;
-  s = set_scene_table(c, {sc});
+  return set_scene_table(c, {sc});
+}
+
+aemh_status single_scene(aemh_ctx* c, float total_dt, int vel_only) {
+  if (c->cfg.substeps == 0) return fail(c, AEMH_ERR_ARG, "substeps must be >= 1");
+  aemh_status s = prepare_single_scene(c, total_dt / (float)c->cfg.substeps,  // mpm_solver.rs:1075
+                                       vel_only);
   if (s != AEMH_OK) return s;
   return run_substeps(c, c->cfg.substeps);
 }
@@ -838,9 +860,113 @@ aemh_status aemh_batch_step(aemh_ctx* c, uint32_t n_substeps) {
   return run_substeps(c, n_substeps);
 }
 
+// Times the candidate launch shapes of P2G (LDS tiles per workgroup), the
+// grid update (workgroup count) and G2P (workgroup size) on the current
+// particles and scenes and keeps the fastest of each.  The substep kernels
+// run without advancing the state: G2P writes the idle ping-pong buffer and
+// the sort and grid buffers are rebuilt by the next step.
 aemh_status aemh_autotune(aemh_ctx* c) {
   if (!c) return AEMH_ERR_ARG;
+  if (!c->mats_d) return fail(c, AEMH_ERR_STATE, "materials not set");
+  hipSetDevice(c->device);
+  aemh_status s = AEMH_OK;
+  if (!c->batch_mode) s = prepare_single_scene(c, c->scd_h.empty() ? 1e-4f : c->scd_h[0].dt, 0);
+  if (s != AEMH_OK) return s;
+  if (c->n_work == 0) return AEMH_OK;
+  const bool prof = c->prof;
+  c->prof = false;
+  const PBuf a = c->work[c->cur].pb, b = c->work[c->cur ^ 1].pb;
+  const uint32_t n = c->n_work;
+  const int ns = (int)c->scd_h.size();
+  const int storage = c->work[0].storage;
+  const int q32 = c->cfg.accum == AEMH_ACCUM_Q32_32;
+  hipStream_t st = c->stream;
+  LaunchShape& ls = c->shape;
+  launch_step_begin(st, c->sort);
+  launch_count(st, a, c->scd_d, ns, n, c->sort);
+  launch_scan(st, c->scd_d, ns, c->sort, c->nb_total);
+  launch_scatter(st, n, c->sort);
+  CHECK_LAUNCH(c);
+  constexpr int kReps = 3;
+  float best = 1e30f, ms = 0.0f;
+  int pick = ls.p2g_tiles;
+  for (int tiles = 2; tiles <= 4; tiles++) {
+    ls.p2g_tiles = tiles;
+    s = time_launches(c, kReps, &ms, [&] {
+      launch_clear(st, c->sort, c->grid, q32, ls.upd_wgs);
+      if (q32)
+        det::launch_p2g(st, ls, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
+      else
+        fast::launch_p2g(st, ls, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
+    });
+    if (s != AEMH_OK) return s;
+    if (ms < best) best = ms, pick = tiles;
+  }
+  ls.p2g_tiles = pick;
+  c->tune_ms[0] = best;
+  best = 1e30f;
+  pick = ls.upd_wgs;
+  for (int wgs : {512, 1024, 2048, 4096}) {
+    ls.upd_wgs = wgs;
+    s = time_launches(c, kReps, &ms, [&] {
+      if (q32)
+        det::launch_grid_update(st, ls, c->scd_d, ns, c->sort, c->grid);
+      else
+        fast::launch_grid_update(st, ls, c->scd_d, ns, c->sort, c->grid);
+    });
+    if (s != AEMH_OK) return s;
+    if (ms < best) best = ms, pick = wgs;
+  }
+  ls.upd_wgs = pick;
+  c->tune_ms[1] = best;
+  best = 1e30f;
+  pick = ls.g2p_wg;
+  for (int wg : {64, 128, 256}) {
+    ls.g2p_wg = wg;
+    s = time_launches(c, kReps, &ms, [&] {
+      if (q32)
+        det::launch_g2p(st, ls, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
+      else
+        fast::launch_g2p(st, ls, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
+    });
+    if (s != AEMH_OK) return s;
+    if (ms < best) best = ms, pick = wg;
+  }
+  ls.g2p_wg = pick;
+  c->tune_ms[2] = best;
+  // The timed grid updates may have raised the Q32.32 range flag on sums
+  // that no step will use; the next step recomputes it.
+  HIPTRY(c, hipMemsetAsync(c->sort.counters + kCntOverflow, 0, 4, st));
+  HIPTRY(c, hipStreamSynchronize(st));
+  c->prof = prof;
+  invalidate_graphs(c);
   c->tuned = true;
+  return AEMH_OK;
+}
+
+aemh_status aemh_get_launch_shape(aemh_ctx* c, uint32_t* p2g_tiles, uint32_t* g2p_wg,
+                                  uint32_t* upd_wgs, float* tuned_ms3) {
+  if (!c) return AEMH_ERR_ARG;
+  if (p2g_tiles) *p2g_tiles = (uint32_t)c->shape.p2g_tiles;
+  if (g2p_wg) *g2p_wg = (uint32_t)c->shape.g2p_wg;
+  if (upd_wgs) *upd_wgs = (uint32_t)c->shape.upd_wgs;
+  if (tuned_ms3)
+    for (int i = 0; i < 3; i++) tuned_ms3[i] = c->tuned ? c->tune_ms[i] : 0.0f;
+  return AEMH_OK;
+}
+
+aemh_status aemh_set_launch_shape(aemh_ctx* c, uint32_t p2g_tiles, uint32_t g2p_wg,
+                                  uint32_t upd_wgs) {
+  if (!c) return AEMH_ERR_ARG;
+  if (p2g_tiles < 2 || p2g_tiles > 4) return fail(c, AEMH_ERR_ARG, "p2g_tiles must be 2, 3 or 4");
+  if (g2p_wg != 64 && g2p_wg != 128 && g2p_wg != 256)
+    return fail(c, AEMH_ERR_ARG, "g2p_wg must be 64, 128 or 256");
+  if (upd_wgs < 1 || upd_wgs > 65536) return fail(c, AEMH_ERR_ARG, "upd_wgs must be in [1, 65536]");
+  c->shape.p2g_tiles = (int)p2g_tiles;
+  c->shape.g2p_wg = (int)g2p_wg;
+  c->shape.upd_wgs = (int)upd_wgs;
+  c->tuned = true;
+  invalidate_graphs(c);
   return AEMH_OK;
 }
 

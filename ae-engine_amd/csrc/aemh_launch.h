@@ -9,10 +9,12 @@
 
 namespace aemh {
 
+// Launch parameters the autotuner chooses (defaults: the 1M-particle snow
+// workload's best).  P2G always runs 256 particles per workgroup.
 struct LaunchShape {
-  int p2g_wg = 256;      // particles per P2G workgroup (one per thread)
-  int g2p_wg = 256;      // G2P workgroup size
-  int upd_wgs = 2048;    // persistent grid-update workgroups (4 blocks each)
+  int p2g_tiles = 3;     // LDS tiles per P2G workgroup: 2, 3 or 4
+  int g2p_wg = 256;      // G2P workgroup size: 64, 128 or 256
+  int upd_wgs = 2048;    // persistent grid-update / clear workgroups
 };
 
 // ---- k_common.hip --------------------------------------------------------

@@ -23,11 +23,12 @@ namespace aemh {
 namespace AEMH_NS {
 namespace {
 
-// Up to this many distinct base blocks of one workgroup's particles are
-// staged in LDS; the rare rest scatter straight to HBM.  3 tiles (37 KB of
-// int64 FP32-mode accumulators) fit four workgroups per CU; 4 tiles fit three
-// and ran P2G 10-20% slower on the 1M and 16M workloads.
-constexpr int kMaxTiles = 3;
+// P2G stages up to TILES (2, 3 or 4; the autotuner picks) distinct base
+// blocks of one workgroup's particles in LDS; the rest scatter straight to
+// HBM.  Each tile holds kTileVals int64 accumulators (12 KB in the FP32 mode,
+// 13.5 KB in the deterministic mode), so the count trades staging coverage
+// against workgroups per CU: dense scenes (~300 particles per block) rarely
+// need a third tile, 2 particles per cell needs three.
 #if AEMH_DET
 constexpr int kTileFields = kDetFields;  // 7 sums + the magnitude bound
 #else
@@ -141,20 +142,20 @@ __device__ __forceinline__ void tile_node_of(int t, int& lx, int& ly, int& lz) {
 // 2^-50 of the group's largest contribution.  Tiles are flushed block-
 // coalesced with one HBM atomic per non-zero node value.  The LDS and HBM
 // scatter paths are separate code so the LDS one stays ds_add.
-template <int STORE>
+template <int STORE, int TILES>
 __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict__ sc, int ns,
                                              const MatDev* __restrict__ mats, int nmat,
                                              uint32_t n, SortDev s, GridDev G) {
-  __shared__ long long tile[kMaxTiles * kTileVals];
+  __shared__ long long tile[TILES * kTileVals];
   __shared__ uint32_t skey[256];
   __shared__ uint32_t sperm[256];
-  __shared__ uint32_t ccnt[kMaxTiles * 64];
+  __shared__ uint32_t ccnt[TILES * 64];
   __shared__ uint32_t lcnt[kLayers];
   __shared__ int16_t sslot[256];
   __shared__ uint8_t order[256];
-  __shared__ int torg[kMaxTiles][3];
-  __shared__ int tdim[kMaxTiles][5];  // scene grid n[0..2], nb[0..1] of the tile
-  __shared__ uint32_t tboff[kMaxTiles];
+  __shared__ int torg[TILES][3];
+  __shared__ int tdim[TILES][5];  // scene grid n[0..2], nb[0..1] of the tile
+  __shared__ uint32_t tboff[TILES];
   __shared__ uint32_t wcnt[4];
   __shared__ int wexp[4][3];
 
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
     const unsigned long long hb = __ballot(head);
     const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
     if (lane == 0) wcnt[wave] = (uint32_t)__popcll(hb);
-    for (int e = t; e < kMaxTiles * 64; e += 256) ccnt[e] = 0;
+    for (int e = t; e < TILES * 64; e += 256) ccnt[e] = 0;
     __syncthreads();
     uint32_t pre = 0, tot = 0;
 #pragma unroll
@@ -188,14 +189,14 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
       tot += wcnt[w];
     }
     slot = (int)(pre + (uint32_t)__popcll(hb & le)) - 1;
-    ntiles = (int)min(tot, (uint32_t)kMaxTiles);
+    ntiles = (int)min(tot, (uint32_t)TILES);
     Stencil st = {{0, 0, 0}, {0.f, 0.f, 0.f}};
     int sid = 0;
     if (valid) {
       sid = scene_of(sc, ns, j);
       st = make_stencil(sc[sid], P.x[0][p], P.x[1][p], P.x[2][p]);
     }
-    if (head && slot < kMaxTiles) {
+    if (head && slot < TILES) {
       const SceneDev& T = sc[sid];
 #pragma unroll
       for (int a = 0; a < 3; a++) {
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
     // base cell.  Lanes are re-assigned layer by layer, so the 64 lanes of one
     // LDS atomic hit distinct nodes (the first kLayers - 1 layers). ----
     int layer = kLayers - 1;
-    if (valid && slot < kMaxTiles) {
+    if (valid && slot < TILES) {
       int c = 0;
 #pragma unroll
       for (int a = 2; a >= 0; a--) c = c * 4 + min(max(st.g[a] - torg[slot][a], 0), 3);
@@ -335,9 +336,9 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
     const int n0 = S.n[0], n1 = S.n[1], n2 = S.n[2];
     const float dx = S.dx, o0 = S.o[0], o1 = S.o[1], o2 = S.o[2];
 #if AEMH_DET
-    const bool lds_path = slot < kMaxTiles;
+    const bool lds_path = slot < TILES;
 #else
-    const bool lds_path = slot < kMaxTiles && finite_in;
+    const bool lds_path = slot < TILES && finite_in;
 #endif
     if (lds_path) {
       // A base cell inside this tile's block reaches local nodes 0..5 only.
@@ -516,7 +517,7 @@ template <int STORE>
 __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev* __restrict__ sc,
                                              int ns, const MatDev* __restrict__ mats, int nmat,
                                              uint32_t n, SortDev s, GridDev G) {
-  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t p = s.perm[j];
   const SceneDev S = sc[scene_of(sc, ns, j)];
@@ -742,16 +743,27 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
 
 }  // namespace
 
-void launch_p2g(hipStream_t st, const LaunchShape&, PBuf cur, const SceneDev* sc, int ns,
+template <int STORE>
+void launch_p2g_store(hipStream_t st, int tiles, unsigned grid, PBuf cur, const SceneDev* sc,
+                      int ns, const MatDev* mats, int nmat, uint32_t n, SortDev s, GridDev g) {
+  if (tiles <= 2)
+    k_p2g<STORE, 2><<<grid, 256, 0, st>>>(cur, sc, ns, mats, nmat, n, s, g);
+  else if (tiles == 3)
+    k_p2g<STORE, 3><<<grid, 256, 0, st>>>(cur, sc, ns, mats, nmat, n, s, g);
+  else
+    k_p2g<STORE, 4><<<grid, 256, 0, st>>>(cur, sc, ns, mats, nmat, n, s, g);
+}
+
+void launch_p2g(hipStream_t st, const LaunchShape& ls, PBuf cur, const SceneDev* sc, int ns,
                 const MatDev* mats, int nmat, uint32_t n, SortDev s, GridDev g, int storage) {
   if (!n) return;
   const unsigned grid = (n + 255u) / 256u;
   if (storage == 1)
-    k_p2g<1><<<grid, 256, 0, st>>>(cur, sc, ns, mats, nmat, n, s, g);
+    launch_p2g_store<1>(st, ls.p2g_tiles, grid, cur, sc, ns, mats, nmat, n, s, g);
   else if (storage == 2)
-    k_p2g<2><<<grid, 256, 0, st>>>(cur, sc, ns, mats, nmat, n, s, g);
+    launch_p2g_store<2>(st, ls.p2g_tiles, grid, cur, sc, ns, mats, nmat, n, s, g);
   else
-    k_p2g<0><<<grid, 256, 0, st>>>(cur, sc, ns, mats, nmat, n, s, g);
+    launch_p2g_store<0>(st, ls.p2g_tiles, grid, cur, sc, ns, mats, nmat, n, s, g);
 }
 
 void launch_grid_update(hipStream_t st, const LaunchShape& ls, const SceneDev* sc, int ns,
@@ -759,17 +771,18 @@ void launch_grid_update(hipStream_t st, const LaunchShape& ls, const SceneDev* s
   k_grid_update<<<ls.upd_wgs, 256, 0, st>>>(sc, ns, s, g);
 }
 
-void launch_g2p(hipStream_t st, const LaunchShape&, PBuf cur, PBuf nxt, const SceneDev* sc,
+void launch_g2p(hipStream_t st, const LaunchShape& ls, PBuf cur, PBuf nxt, const SceneDev* sc,
                 int ns, const MatDev* mats, int nmat, uint32_t n, SortDev s, GridDev g,
                 int storage) {
   if (!n) return;
-  const unsigned grid = (n + 255u) / 256u;
+  const unsigned wg = (unsigned)ls.g2p_wg;
+  const unsigned grid = (n + wg - 1u) / wg;
   if (storage == 1)
-    k_g2p<1><<<grid, 256, 0, st>>>(cur, nxt, sc, ns, mats, nmat, n, s, g);
+    k_g2p<1><<<grid, wg, 0, st>>>(cur, nxt, sc, ns, mats, nmat, n, s, g);
   else if (storage == 2)
-    k_g2p<2><<<grid, 256, 0, st>>>(cur, nxt, sc, ns, mats, nmat, n, s, g);
+    k_g2p<2><<<grid, wg, 0, st>>>(cur, nxt, sc, ns, mats, nmat, n, s, g);
   else
-    k_g2p<0><<<grid, 256, 0, st>>>(cur, nxt, sc, ns, mats, nmat, n, s, g);
+    k_g2p<0><<<grid, wg, 0, st>>>(cur, nxt, sc, ns, mats, nmat, n, s, g);
 }
 
 }  // namespace AEMH_NS

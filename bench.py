@@ -56,6 +56,8 @@ def run_main(W, aemh, steps, warmup, accum=None):
         wl.accum = accum
     eng = aemh.Engine()
     W.load_engine(eng, wl)
+    eng.step(wl.dt)
+    shape = eng.autotune()  # one-time launch-shape choice, before the timed region
     for _ in range(warmup):
         eng.step(wl.dt)
     eng.profile(True)
@@ -66,7 +68,7 @@ def run_main(W, aemh, steps, warmup, accum=None):
     blocks, npart = eng.last_substep_info()
     el_graph = timed_steps(eng, wl.dt, steps)
     eng.close()
-    return wl, el, el_graph, kern, blocks, npart
+    return wl, el, el_graph, kern, blocks, npart, shape
 
 
 def roofline_of(kern, name, npart, blocks):
@@ -110,7 +112,9 @@ def extra_c3(W, aemh, steps=10):
     wl = W.c3_three_boxes()
     eng = aemh.Engine()
     W.load_engine(eng, wl)
-    for _ in range(3):
+    eng.step(wl.dt)
+    shape = eng.autotune()
+    for _ in range(2):
         eng.step(wl.dt)
     el = timed_steps(eng, wl.dt, steps)
     eng.profile(True)
@@ -131,7 +135,41 @@ def extra_c3(W, aemh, steps=10):
         "g2p": roofline_of(kern, "g2p", npart, blocks),
         "p2g": roofline_of(kern, "p2g", npart, blocks),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
+        "launch_shape": shape,
     }
+
+
+def extra_c5(W, aemh, steps=20):
+    """4,194,304 particles on 192^3, C and F - I stored in BF16 (FP32 compute),
+    FP64 energy/momentum reductions every 10 steps inside the timed loop; the
+    same run with FP32 storage beside it."""
+    out = {"workload": "c5_mixed_4m_192: 4,194,304 particles (snow + Neo-Hookean boxes), 192^3 "
+                       "grid, dt 5e-5, C and F-I in BF16 / FP32 storage, FP32 compute, FP64 "
+                       "conservation sums every 10 steps"}
+    for name, storage in (("bf16", aemh.STORE_BF16), ("f32", aemh.STORE_F32)):
+        wl = W.c5_mixed_precision(storage=storage)
+        eng = aemh.Engine()
+        W.load_engine(eng, wl)
+        eng.step(wl.dt)
+        eng.autotune()
+        eng.step(wl.dt)
+        sums = []
+        t0 = time.perf_counter()
+        for k in range(steps):
+            eng.step(wl.dt)
+            if k % 10 == 9:
+                sums.append(eng.conservation_f64())
+        el = time.perf_counter() - t0
+        eng.profile(True)
+        eng.profile_reset()
+        timed_steps(eng, wl.dt, 3)
+        kern = {k: eng.profile_get(k)[0] for k in KERNELS}
+        eng.close()
+        out[name] = {"particle_updates_per_s": wl.n * steps / el, "ms_per_step": el / steps * 1e3,
+                     "p2g_ms": round(kern["p2g"], 4), "g2p_ms": round(kern["g2p"], 4),
+                     "fp64_sums_last": [float(x) for x in sums[-1]]}
+    out["bf16_vs_f32_speedup"] = out["bf16"]["particle_updates_per_s"] / out["f32"]["particle_updates_per_s"]
+    return out
 
 
 def extra_batched(W, aemh, steps=5):
@@ -186,7 +224,7 @@ def main():
     from aemh import workloads as W
 
     log("[bench] main workload: C2 1M snow particles, 128^3")
-    wl, el, el_graph, kern, blocks, npart = run_main(W, aemh, args.steps, args.warmup)
+    wl, el, el_graph, kern, blocks, npart, shape = run_main(W, aemh, args.steps, args.warmup)
     value = wl.n * args.steps / el
     dominant = max(("p2g", "g2p", "grid_update"), key=lambda k: kern[k])
     step_bytes = npart * BYTES["step"][0] + blocks * 64 * BYTES["step"][1]
@@ -205,7 +243,8 @@ def main():
         "data": "synthetic: seeded xoshiro256** particles (seed 42+index), v=0, F=I",
         "config": {"workload": "c2_snow_1m_128: 1,048,576 snow particles in a dam-break block, "
                                "128^3 grid, dx 1/128, dt 5e-5, FP32 P2G",
-                   "particles": wl.n, "grid": [128, 128, 128], "accum": "f32", "substeps": 1},
+                   "particles": wl.n, "grid": [128, 128, 128], "accum": "f32", "substeps": 1,
+                   "launch_shape": {k: shape[k] for k in ("p2g_tiles", "g2p_wg", "upd_wgs")}},
         "roofline": roofline_of(kern, dominant, npart, blocks),
         "step_hbm_frac": round(step_bytes / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
@@ -215,13 +254,15 @@ def main():
              "active_blocks": blocks}
     if not args.no_extras:
         log("[bench] deterministic Q32.32 mode")
-        wq, elq, elq_g, kq, _, _ = run_main(W, aemh, max(5, args.steps // 5), 2,
-                                              accum=aemh.ACCUM_Q32_32)
+        wq, elq, elq_g, kq, _, _, _ = run_main(W, aemh, max(5, args.steps // 5), 2,
+                                                 accum=aemh.ACCUM_Q32_32)
         extra["q32_particle_updates_per_s"] = wq.n * max(5, args.steps // 5) / elq
         log("[bench] C3 16M particles")
         extra["c3"] = extra_c3(W, aemh)
         log("[bench] C4 batched vs plain")
         extra["c4"] = extra_batched(W, aemh)
+        log("[bench] C5 mixed precision")
+        extra["c5"] = extra_c5(W, aemh)
     line["extra"] = extra
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")

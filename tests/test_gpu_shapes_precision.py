@@ -1,0 +1,139 @@
+"""GPU: launch-shape independence, the autotuner, and 16-bit storage of C and
+F measured against the FP64 oracle.
+
+Launch shapes change which particles take the LDS-tile path and which the
+direct HBM-atomic path in P2G (2 tiles per workgroup sends the most particles
+down the latter), the grid-update work split and the G2P workgroup size; the
+deterministic mode must stay bit-exact with the oracle under every shape.
+"""
+import numpy as np
+import pytest
+
+import aemh
+from aemh import workloads as W
+from oracle import refcpu as R
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("pos", "vel", "C", "F", "J", "Jp")
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def oracle_run(wl, steps, dtype=np.float32, threads=4):
+    nx, ny, nz, dx, o = wl.grid
+    acc = R.ACCUM_Q32 if wl.accum == aemh.ACCUM_Q32_32 else R.ACCUM_F32
+    cfg = R.make_config(nx, ny, nz, dx, o, 1, wl.gravity, 0.995, wl.bc_margin, 0.0, acc,
+                        R.CLAMP_AUTO, threads=threads)
+    p, m, e = aemh.material_arrays(wl.materials)
+    M = R.Materials(p, m, e)
+    st = R.State(wl.pos, mass=wl.mass, mat=wl.mat, dtype=dtype)
+    for _ in range(steps):
+        R.step(cfg, M, st, wl.dt)
+    return st
+
+
+def state_of(st):
+    return dict(pos=st.pos, vel=st.vel, C=st.C, F=st.F, J=st.J, Jp=st.Jp)
+
+
+def assert_bitexact(d, st):
+    want = state_of(st)
+    for f in FIELDS:
+        g = d[f].reshape(want[f].shape)
+        assert np.array_equal(g, want[f]), f"{f}: {int((g != want[f]).sum())} values differ"
+
+
+@pytest.fixture(scope="module")
+def sparse_snow():
+    # ~1 particle per cell: a 256-particle workgroup spans ~4-5 base blocks,
+    # so every tile budget leaves some particles on the HBM path.
+    wl = W.c2_snow_dam(n=1 << 16, steps=10, accum=aemh.ACCUM_Q32_32, res=64)
+    return wl, oracle_run(wl, 10)
+
+
+@pytest.mark.parametrize("shape", [(2, 64, 512), (3, 128, 1024), (4, 256, 4096)])
+def test_launch_shapes_deterministic_bitexact(sparse_snow, shape):
+    wl, st = sparse_snow
+    eng = aemh.Engine()
+    W.load_engine(eng, wl)
+    eng.set_launch_shape(*shape)
+    assert eng.launch_shape()["p2g_tiles"] == shape[0]
+    for _ in range(10):
+        eng.step(wl.dt)
+    assert_bitexact(eng.download(), st)
+    eng.close()
+
+
+def test_autotune_does_not_advance_state():
+    wl = W.c1_elastic_cube()
+    wl.accum = aemh.ACCUM_Q32_32
+    st = oracle_run(wl, 10)
+    eng = aemh.Engine()
+    W.load_engine(eng, wl)
+    for _ in range(5):
+        eng.step(wl.dt)
+    shape = eng.autotune()
+    assert shape["p2g_tiles"] in (2, 3, 4)
+    assert shape["g2p_wg"] in (64, 128, 256)
+    assert shape["upd_wgs"] in (512, 1024, 2048, 4096)
+    assert all(t > 0 for t in shape["tuned_ms"])
+    for _ in range(5):
+        eng.step(wl.dt)
+    assert_bitexact(eng.download(), st)
+    eng.close()
+
+
+def test_launch_shape_rejects_bad_values():
+    eng = aemh.Engine()
+    for bad in [(1, 256, 2048), (5, 256, 2048), (3, 96, 2048), (3, 256, 0)]:
+        with pytest.raises(aemh.AemhError):
+            eng.set_launch_shape(*bad)
+    eng.close()
+
+
+def _c5_scaled(storage, n=1 << 16, res=48):
+    """The mixed-precision workload (snow box + Neo-Hookean box) at 65,536
+    particles on 48^3, keeping its ~5.5 particles per occupied cell."""
+    return W.c5_mixed_precision(n=n, res=res, storage=storage)
+
+
+def test_bf16_f16_storage_error_vs_fp64_oracle():
+    """50 steps; x, v, F and displacement against the FP64 oracle.
+
+    FP32 storage sets the floor: FP32 arithmetic against FP64 (measured on
+    MI355X: v 9.3e-4, F 3.6e-7, displacement 1.4e-2 -- the displacement
+    after 2.5 ms is ~3e-5 while an FP32 position near 0.5 rounds at 6e-8).
+    BF16 and FP16 keep C and F - I in 16 bits with FP32 compute (measured:
+    v 9.2e-4 / 9.6e-4, F 6.5e-7 / 4.3e-7).  Bounds: positions 1e-6, v 5e-3,
+    F 1e-5 for every storage; 16-bit storage within 1.5x of the FP32 floor
+    on v and displacement, and within 5e-6 on F."""
+    steps = 50
+    wl64 = _c5_scaled(aemh.STORE_F32)
+    ref = oracle_run(wl64, steps, dtype=np.float64, threads=8)
+    x0 = wl64.pos.astype(np.float64)
+    err = {}
+    for name, storage in (("f32", aemh.STORE_F32), ("bf16", aemh.STORE_BF16), ("f16", aemh.STORE_F16)):
+        wl = _c5_scaled(storage)
+        eng = aemh.Engine()
+        W.load_engine(eng, wl)
+        for _ in range(steps):
+            eng.step(wl.dt)
+        d = eng.download()
+        eng.close()
+        err[name] = dict(pos=rel(d["pos"], ref.pos), vel=rel(d["vel"], ref.vel),
+                         disp=rel(d["pos"] - x0, ref.pos - x0), F=rel(d["F"], ref.F))
+    print("\nstorage error vs FP64 oracle after 50 steps:", err)
+    for name in err:
+        assert err[name]["pos"] < 1e-6, (name, err[name])
+        assert err[name]["vel"] < 5e-3, (name, err[name])
+        assert err[name]["F"] < 1e-5, (name, err[name])
+    f32 = err["f32"]
+    for name in ("bf16", "f16"):
+        assert err[name]["vel"] <= 1.5 * f32["vel"], (name, err)
+        assert err[name]["disp"] <= 1.5 * f32["disp"], (name, err)
+        assert err[name]["F"] <= f32["F"] + 5e-6, (name, err)
