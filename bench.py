@@ -37,6 +37,10 @@ HBM_PEAK_GBS = 8000.0
 #   16 (v, m); G2P reads 16 -> step 88 B; P2G 28 B; G2P 16 B.
 BYTES = {"p2g": (110, 28), "g2p": (156, 16), "grid_update": (0, 44), "step": (214, 88)}
 KERNELS = ["step_begin", "count", "scan", "scatter", "clear", "p2g", "grid_update", "g2p"]
+# HBM bytes per dispatch on the 1M-particle snow workload, from rocprofv3
+# FETCH_SIZE (x 2.000) and WRITE_SIZE (x 1.000) against a 256 MiB copy, as
+# recorded in profiles/c2_snow_1m_hbm_traffic.md for this build's kernels.
+TRAFFIC_C2 = {"p2g": 408.2e6, "g2p": 224.5e6, "grid_update": 26.5e6}
 
 
 def log(msg):
@@ -71,14 +75,15 @@ def run_main(W, aemh, steps, warmup, accum=None):
     return wl, el, el_graph, kern, blocks, npart, shape
 
 
-def roofline_of(kern, name, npart, blocks):
+def roofline_of(kern, name, npart, blocks, traffic=None):
     per_p, per_n = BYTES[name]
     nbytes = npart * per_p + blocks * 64 * per_n
     ms = kern[name]
     achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
-            "algorithmic_bytes": int(nbytes), "avg_ms": round(ms, 4)}
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None if traffic is None else int(traffic.get(name, 0)) or None,
+            "kernel": name, "algorithmic_bytes": int(nbytes), "avg_ms": round(ms, 4)}
 
 
 def cpu_baseline(W, aemh):
@@ -207,6 +212,27 @@ def extra_batched(W, aemh, steps=5):
             "batched_vs_plain_speedup": el_p / el_b}
 
 
+def probe(W, aemh):
+    """Counter-collection run.  The copy moves exactly 256 MiB in and 256 MiB
+    out, so FETCH_SIZE / WRITE_SIZE of its dispatch give the KiB each counter
+    reports per real KiB; the P2G / G2P dispatches of the 5 steps follow."""
+    import torch
+    x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    wl = W.c2_snow_dam()
+    eng = aemh.Engine()
+    W.load_engine(eng, wl)
+    eng.step(wl.dt)
+    eng.set_launch_shape(3, 256, 2048)
+    for _ in range(5):
+        eng.step(wl.dt)
+    eng.close()
+    print("probe done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +240,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="short run for hardware-counter collection: a 256 MiB device copy "
+                         "(calibrates FETCH_SIZE / WRITE_SIZE) then 5 steps of the main workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: every box here has one GPU and the MPM step does "
@@ -222,6 +251,10 @@ def main():
         sys.exit(2)
     import aemh
     from aemh import workloads as W
+
+    if args.probe:
+        probe(W, aemh)
+        return
 
     log("[bench] main workload: C2 1M snow particles, 128^3")
     wl, el, el_graph, kern, blocks, npart, shape = run_main(W, aemh, args.steps, args.warmup)
@@ -245,12 +278,12 @@ def main():
                                "128^3 grid, dx 1/128, dt 5e-5, FP32 P2G",
                    "particles": wl.n, "grid": [128, 128, 128], "accum": "f32", "substeps": 1,
                    "launch_shape": {k: shape[k] for k in ("p2g_tiles", "g2p_wg", "upd_wgs")}},
-        "roofline": roofline_of(kern, dominant, npart, blocks),
+        "roofline": roofline_of(kern, dominant, npart, blocks, TRAFFIC_C2),
         "step_hbm_frac": round(step_bytes / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
     }
     extra = {"graph_replay_particle_updates_per_s": wl.n * args.steps / el_graph,
-             "g2p_roofline": roofline_of(kern, "g2p", npart, blocks),
+             "g2p_roofline": roofline_of(kern, "g2p", npart, blocks, TRAFFIC_C2),
              "active_blocks": blocks}
     if not args.no_extras:
         log("[bench] deterministic Q32.32 mode")
