@@ -340,6 +340,36 @@ def test_snow_clamp_closed_form():
         assert abs(jp - jp0 * np.prod(sv) / np.prod(sh)) < 2e-5 * max(1, jp)
 
 
+def test_snow_clamp_skip_rule_against_numpy_svd():
+    """F and J_p stay bit-identical exactly when no singular value leaves
+    [1 - theta_c, 1 + theta_s] (the Gershgorin shortcut must never skip a
+    clamp); otherwise F = U clamp(S) V^T and J_p scales by prod(s / s_hat).
+    Random F around the yield surface, checked against numpy's FP64 SVD;
+    cases within 1e-5 of a bound are left out (float rounding decides)."""
+    rng = np.random.default_rng(5)
+    lo, hi = 1 - 2.5e-2, 1 + 7.5e-3
+    n_skip = n_clamp = 0
+    for _ in range(400):
+        sv = rng.uniform(0.96, 1.02, 3)
+        F = (_rot(rng.normal(size=3), rng.uniform(0, 180)) @ np.diag(sv)
+             @ _rot(rng.normal(size=3), rng.uniform(0, 180)).T).astype(f32)
+        s = np.linalg.svd(F.astype(np.float64))[1]
+        if np.min(np.abs(np.concatenate([s - lo, s - hi]))) < 1e-5:
+            continue
+        Fc, jp = R.snow_clamp(F, 1.0)
+        inside = bool(np.all((s >= lo) & (s <= hi)))
+        if inside:
+            n_skip += 1
+            assert np.array_equal(Fc, F) and jp == 1.0, sv
+        else:
+            n_clamp += 1
+            U, s2, Vt = np.linalg.svd(F.astype(np.float64))
+            sh = np.clip(s2, lo, hi)
+            assert np.allclose(Fc, U @ np.diag(sh) @ Vt, atol=3e-6), sv
+            assert abs(jp - np.prod(s2) / np.prod(sh)) < 1e-5
+    assert n_skip > 40 and n_clamp > 40, (n_skip, n_clamp)
+
+
 # ---------------------------------------------------------------------------
 # Step-level closed forms and invariants
 # ---------------------------------------------------------------------------
