@@ -113,8 +113,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.aemh_set_scenes.argtypes = [P, C.POINTER(Scene), u32]
     lib.aemh_batch_step.argtypes = [P, u32]
     lib.aemh_autotune.argtypes = [P]
-    lib.aemh_get_launch_shape.argtypes = [P, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), fp]
-    lib.aemh_set_launch_shape.argtypes = [P, u32, u32, u32]
+    lib.aemh_get_launch_shape.argtypes = [P, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32),
+                                          C.POINTER(u32), fp]
+    lib.aemh_set_launch_shape.argtypes = [P, u32, u32, u32, u32]
     lib.aemh_download_grid.argtypes = [P, u32, fp, fp, fp, fp]
     lib.aemh_download_grid_q32.argtypes = [P, u32, C.POINTER(C.c_int64)]
     lib.aemh_conservation_f64.argtypes = [P, C.POINTER(C.c_double)]
@@ -280,16 +281,18 @@ class Engine:
         return self.launch_shape()
 
     def launch_shape(self):
-        """dict(p2g_tiles, g2p_wg, upd_wgs, tuned_ms=(p2g, update, g2p))."""
-        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        """dict(p2g_tiles, p2g_partials, g2p_wg, upd_wgs,
+        tuned_ms=(p2g + update, update, g2p))."""
+        a, p, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
         ms = np.zeros(3, np.float32)
-        self._check(self._lib.aemh_get_launch_shape(self._h, C.byref(a), C.byref(b), C.byref(c),
-                                                    _f(ms)))
-        return dict(p2g_tiles=a.value, g2p_wg=b.value, upd_wgs=c.value,
+        self._check(self._lib.aemh_get_launch_shape(self._h, C.byref(a), C.byref(p), C.byref(b),
+                                                    C.byref(c), _f(ms)))
+        return dict(p2g_tiles=a.value, p2g_partials=p.value, g2p_wg=b.value, upd_wgs=c.value,
                     tuned_ms=tuple(float(x) for x in ms))
 
-    def set_launch_shape(self, p2g_tiles=3, g2p_wg=256, upd_wgs=2048):
-        self._check(self._lib.aemh_set_launch_shape(self._h, p2g_tiles, g2p_wg, upd_wgs))
+    def set_launch_shape(self, p2g_tiles=3, p2g_partials=1, g2p_wg=256, upd_wgs=2048):
+        self._check(self._lib.aemh_set_launch_shape(self._h, p2g_tiles, p2g_partials, g2p_wg,
+                                                    upd_wgs))
 
     # -- readback ---------------------------------------------------------------
     def download_grid(self, scene=0, nodes=None):

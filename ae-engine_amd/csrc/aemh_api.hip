@@ -79,6 +79,8 @@ struct aemh_ctx {
   uint32_t grid_nb_cap = 0;
   SortDev sort{};
   uint32_t sort_n_cap = 0, sort_nb_cap = 0;
+  size_t part_wg_cap = 0;
+  int part_q32 = 0;
   LaunchShape shape;
   bool tuned = false;
   float tune_ms[3] = {0.0f, 0.0f, 0.0f};  // best P2G, grid update, G2P time
@@ -253,7 +255,10 @@ aemh_status ensure_grid_sort(aemh_ctx* c) {
     if (c->grid.accf) hipFree(c->grid.accf);
     if (c->grid.accq) hipFree(c->grid.accq);
     if (c->grid.gv) hipFree(c->grid.gv);
+    if (c->grid.partf) hipFree(c->grid.partf);
+    if (c->grid.partq) hipFree(c->grid.partq);
     c->grid = GridDev{};
+    c->part_wg_cap = 0;
     const size_t nodes = (size_t)nb * kBlockNodes;
     if (nodes >= (1ull << 32)) return fail(c, AEMH_ERR_ARG, "grid too large: %zu nodes", nodes);
     if (q32) {
@@ -272,7 +277,7 @@ aemh_status ensure_grid_sort(aemh_ctx* c) {
   }
   if (nb > c->sort_nb_cap || !c->sort.cnt) {
     for (void* p : {(void*)c->sort.cnt, (void*)c->sort.off, (void*)c->sort.tsum,
-                    (void*)c->sort.flag, (void*)c->sort.active})
+                    (void*)c->sort.flag, (void*)c->sort.active, (void*)c->sort.bslot})
       if (p) hipFree(p);
     const uint32_t tiles = (nb + kScanTile - 1) / kScanTile;
     HIPTRY(c, hipMalloc(&c->sort.cnt, (size_t)nb * 4));
@@ -283,12 +288,33 @@ aemh_status ensure_grid_sort(aemh_ctx* c) {
     HIPTRY(c, hipMemset(c->sort.flag, 0, (size_t)nb * 4));
     HIPTRY(c, hipMalloc(&c->sort.active, (size_t)nb * 4));
     HIPTRY(c, hipMemset(c->sort.active, 0, (size_t)nb * 4));
+    HIPTRY(c, hipMalloc(&c->sort.bslot, (size_t)nb * 4));
+    HIPTRY(c, hipMemset(c->sort.bslot, 0xff, (size_t)nb * 4));
     c->sort_nb_cap = nb;
     invalidate_graphs(c);
   }
   if (!c->sort.counters) {
     HIPTRY(c, hipMalloc(&c->sort.counters, 64));
     HIPTRY(c, hipMemset(c->sort.counters, 0, 64));
+  }
+  // P2G tile partials: room for the largest tile budget, 216 nodes x fields
+  // per tile, one slot per 256-particle workgroup.
+  {
+    const size_t wgs = ((size_t)(n ? n : 1) + 255) / 256;
+    const size_t vals = wgs * kMaxTileBudget * (size_t)kDetFields * kTileNodes;
+    if (wgs > c->part_wg_cap || q32 != c->part_q32 || (!c->grid.partf && !c->grid.partq)) {
+      if (c->grid.partf) hipFree(c->grid.partf);
+      if (c->grid.partq) hipFree(c->grid.partq);
+      c->grid.partf = nullptr;
+      c->grid.partq = nullptr;
+      if (q32)
+        HIPTRY(c, hipMalloc(&c->grid.partq, vals * sizeof(long long)));
+      else
+        HIPTRY(c, hipMalloc(&c->grid.partf, vals * sizeof(float)));
+      c->part_wg_cap = wgs;
+      c->part_q32 = q32;
+      invalidate_graphs(c);
+    }
   }
   if (n > c->sort_n_cap || !c->sort.key) {
     for (void* p : {(void*)c->sort.key, (void*)c->sort.rank, (void*)c->sort.perm})
@@ -415,7 +441,7 @@ void enqueue_substep(aemh_ctx* c, int cur) {
     timed(c, "p2g", [&] {
       det::launch_p2g(st, c->shape, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
     });
-    timed(c, "grid_update", [&] { det::launch_grid_update(st, c->shape, c->scd_d, ns, c->sort, c->grid); });
+    timed(c, "grid_update", [&] { det::launch_grid_update(st, c->shape, c->scd_d, ns, c->sort, c->grid, n, c->nb_total); });
     timed(c, "g2p", [&] {
       det::launch_g2p(st, c->shape, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
     });
@@ -423,7 +449,7 @@ void enqueue_substep(aemh_ctx* c, int cur) {
     timed(c, "p2g", [&] {
       fast::launch_p2g(st, c->shape, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
     });
-    timed(c, "grid_update", [&] { fast::launch_grid_update(st, c->shape, c->scd_d, ns, c->sort, c->grid); });
+    timed(c, "grid_update", [&] { fast::launch_grid_update(st, c->shape, c->scd_d, ns, c->sort, c->grid, n, c->nb_total); });
     timed(c, "g2p", [&] {
       fast::launch_g2p(st, c->shape, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
     });
@@ -635,7 +661,7 @@ aemh_status aemh_destroy(aemh_ctx* c) {
                   (void*)c->grid.accq, (void*)c->grid.gv, (void*)c->sort.key, (void*)c->sort.rank,
                   (void*)c->sort.perm, (void*)c->sort.cnt, (void*)c->sort.off, (void*)c->sort.tsum,
                   (void*)c->sort.flag, (void*)c->sort.active, (void*)c->sort.counters,
-                  })
+                  (void*)c->sort.bslot, (void*)c->grid.partf, (void*)c->grid.partq})
     if (p) hipFree(p);
   if (c->flag_h) hipHostFree(c->flag_h);
   hipStreamDestroy(c->stream);
@@ -860,9 +886,10 @@ aemh_status aemh_batch_step(aemh_ctx* c, uint32_t n_substeps) {
   return run_substeps(c, n_substeps);
 }
 
-// Times the candidate launch shapes of P2G (LDS tiles per workgroup), the
-// grid update (workgroup count) and G2P (workgroup size) on the current
-// particles and scenes and keeps the fastest of each.  The substep kernels
+// Times the candidate launch shapes of P2G (LDS tiles per workgroup and tile
+// flush mode, with the grid update that consumes it), the grid update
+// (workgroup count) and G2P (workgroup size) on the current particles and
+// scenes and keeps the fastest of each.  The substep kernels
 // run without advancing the state: G2P writes the idle ping-pong buffer and
 // the sort and grid buffers are rebuilt by the next step.
 aemh_status aemh_autotune(aemh_ctx* c) {
@@ -889,30 +916,44 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   CHECK_LAUNCH(c);
   constexpr int kReps = 3;
   float best = 1e30f, ms = 0.0f;
-  int pick = ls.p2g_tiles;
-  for (int tiles = 2; tiles <= 4; tiles++) {
-    ls.p2g_tiles = tiles;
-    s = time_launches(c, kReps, &ms, [&] {
-      launch_clear(st, c->sort, c->grid, q32, ls.upd_wgs);
-      if (q32)
-        det::launch_p2g(st, ls, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
-      else
-        fast::launch_p2g(st, ls, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
-    });
-    if (s != AEMH_OK) return s;
-    if (ms < best) best = ms, pick = tiles;
-  }
+  int pick = ls.p2g_tiles, pick_part = ls.p2g_partials;
+  // Tile budget and flush mode together: the flush mode moves work between
+  // P2G and the grid update, so both are timed.
+  for (int part = 0; part <= 1; part++)
+    for (int tiles = 2; tiles <= 4; tiles++) {
+      ls.p2g_tiles = tiles;
+      ls.p2g_partials = part;
+      s = time_launches(c, kReps, &ms, [&] {
+        launch_clear(st, c->sort, c->grid, q32, ls.upd_wgs);
+        if (q32) {
+          det::launch_p2g(st, ls, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
+          det::launch_grid_update(st, ls, c->scd_d, ns, c->sort, c->grid, n, c->nb_total);
+        } else {
+          fast::launch_p2g(st, ls, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
+          fast::launch_grid_update(st, ls, c->scd_d, ns, c->sort, c->grid, n, c->nb_total);
+        }
+      });
+      if (s != AEMH_OK) return s;
+      if (ms < best) best = ms, pick = tiles, pick_part = part;
+    }
   ls.p2g_tiles = pick;
+  ls.p2g_partials = pick_part;
   c->tune_ms[0] = best;
+  // Partials in the chosen tile layout for the grid-update candidates.
+  launch_clear(st, c->sort, c->grid, q32, ls.upd_wgs);
+  if (q32)
+    det::launch_p2g(st, ls, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
+  else
+    fast::launch_p2g(st, ls, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
   best = 1e30f;
   pick = ls.upd_wgs;
-  for (int wgs : {512, 1024, 2048, 4096}) {
+  for (int wgs : {1024, 2048, 4096, 8192, 16384}) {
     ls.upd_wgs = wgs;
     s = time_launches(c, kReps, &ms, [&] {
       if (q32)
-        det::launch_grid_update(st, ls, c->scd_d, ns, c->sort, c->grid);
+        det::launch_grid_update(st, ls, c->scd_d, ns, c->sort, c->grid, n, c->nb_total);
       else
-        fast::launch_grid_update(st, ls, c->scd_d, ns, c->sort, c->grid);
+        fast::launch_grid_update(st, ls, c->scd_d, ns, c->sort, c->grid, n, c->nb_total);
     });
     if (s != AEMH_OK) return s;
     if (ms < best) best = ms, pick = wgs;
@@ -944,10 +985,11 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   return AEMH_OK;
 }
 
-aemh_status aemh_get_launch_shape(aemh_ctx* c, uint32_t* p2g_tiles, uint32_t* g2p_wg,
-                                  uint32_t* upd_wgs, float* tuned_ms3) {
+aemh_status aemh_get_launch_shape(aemh_ctx* c, uint32_t* p2g_tiles, uint32_t* p2g_partials,
+                                  uint32_t* g2p_wg, uint32_t* upd_wgs, float* tuned_ms3) {
   if (!c) return AEMH_ERR_ARG;
   if (p2g_tiles) *p2g_tiles = (uint32_t)c->shape.p2g_tiles;
+  if (p2g_partials) *p2g_partials = (uint32_t)c->shape.p2g_partials;
   if (g2p_wg) *g2p_wg = (uint32_t)c->shape.g2p_wg;
   if (upd_wgs) *upd_wgs = (uint32_t)c->shape.upd_wgs;
   if (tuned_ms3)
@@ -955,14 +997,16 @@ aemh_status aemh_get_launch_shape(aemh_ctx* c, uint32_t* p2g_tiles, uint32_t* g2
   return AEMH_OK;
 }
 
-aemh_status aemh_set_launch_shape(aemh_ctx* c, uint32_t p2g_tiles, uint32_t g2p_wg,
-                                  uint32_t upd_wgs) {
+aemh_status aemh_set_launch_shape(aemh_ctx* c, uint32_t p2g_tiles, uint32_t p2g_partials,
+                                  uint32_t g2p_wg, uint32_t upd_wgs) {
   if (!c) return AEMH_ERR_ARG;
   if (p2g_tiles < 2 || p2g_tiles > 4) return fail(c, AEMH_ERR_ARG, "p2g_tiles must be 2, 3 or 4");
+  if (p2g_partials > 1) return fail(c, AEMH_ERR_ARG, "p2g_partials must be 0 or 1");
   if (g2p_wg != 64 && g2p_wg != 128 && g2p_wg != 256)
     return fail(c, AEMH_ERR_ARG, "g2p_wg must be 64, 128 or 256");
   if (upd_wgs < 1 || upd_wgs > 65536) return fail(c, AEMH_ERR_ARG, "upd_wgs must be in [1, 65536]");
   c->shape.p2g_tiles = (int)p2g_tiles;
+  c->shape.p2g_partials = (int)p2g_partials;
   c->shape.g2p_wg = (int)g2p_wg;
   c->shape.upd_wgs = (int)upd_wgs;
   c->tuned = true;

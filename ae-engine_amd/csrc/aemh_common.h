@@ -21,6 +21,7 @@ constexpr int kBlockNodes = 64;
 // P2G stages the 6x6x6 node halo of a 4x4x4 block of base cells in LDS.
 constexpr int kTileSide = 6;
 constexpr int kTileNodes = 216;
+constexpr int kMaxTileBudget = 4;  // most LDS tiles a P2G workgroup may use
 
 // One scene as seen by the kernels (derived from aemh_scene + aemh_config).
 struct SceneDev {
@@ -74,6 +75,10 @@ struct GridDev {
   long long* accq;  // [8][nodes] Q32.32 raw sums (deterministic mode); field 7
                     // bounds sum |contribution| in units of 2^32 raw
   float4* gv;       // [nodes] (v.xyz, m) after the grid update
+  // P2G tile partials, [workgroup][tile][field][216 tile nodes]: FP32 sums
+  // (FP32 mode) or Q32.32 raw sums (deterministic mode)
+  float* partf;
+  long long* partq;
   uint32_t nodes;
 };
 constexpr int kDetFields = 8;
@@ -93,6 +98,8 @@ struct SortDev {
   uint32_t* flag;       // [NB] node block active this substep
   uint32_t* active;     // [NB] list of active node blocks
   uint32_t* counters;   // [8], see kCnt*
+  uint32_t* bslot;      // [NB] tile slot of the block in the P2G workgroup
+                        // holding its first particle (>= tiles: no tile)
 };
 
 constexpr int kScanTile = 1024;

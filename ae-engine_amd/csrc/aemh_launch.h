@@ -13,6 +13,8 @@ namespace aemh {
 // workload's best).  P2G always runs 256 particles per workgroup.
 struct LaunchShape {
   int p2g_tiles = 3;     // LDS tiles per P2G workgroup: 2, 3 or 4
+  int p2g_partials = 1;  // P2G tile flush: 1 partial stores gathered by the
+                         // grid update, 0 one HBM atomic per node value
   int g2p_wg = 256;      // G2P workgroup size: 64, 128 or 256
   int upd_wgs = 2048;    // persistent grid-update / clear workgroups
 };
@@ -44,7 +46,7 @@ void launch_conservation(hipStream_t st, PBuf cur, uint32_t n, double* out5);
                   int ns, const MatDev* mats, int nmat, uint32_t n, SortDev s, GridDev g,   \
                   int storage);                                                             \
   void launch_grid_update(hipStream_t st, const LaunchShape& ls, const SceneDev* sc,        \
-                          int ns, SortDev s, GridDev g);                                    \
+                          int ns, SortDev s, GridDev g, uint32_t n, uint32_t nb_total);     \
   void launch_g2p(hipStream_t st, const LaunchShape& ls, PBuf cur, PBuf nxt,                \
                   const SceneDev* sc, int ns, const MatDev* mats, int nmat, uint32_t n,     \
                   SortDev s, GridDev g, int storage);                                       \

@@ -35,6 +35,9 @@ constexpr int kTileFields = kDetFields;  // 7 sums + the magnitude bound
 constexpr int kTileFields = 7;
 #endif
 constexpr int kTileVals = kTileFields * kTileNodes;
+// Values per tile-node record of the P2G partials (7 sums + pad, or the 8
+// deterministic-mode fields).
+constexpr int kPartStride = 8;
 // Lane layers of the P2G re-assignment (rank within a base cell, capped).
 constexpr int kLayers = 16;
 
@@ -145,7 +148,7 @@ __device__ __forceinline__ void tile_node_of(int t, int& lx, int& ly, int& lz) {
 template <int STORE, int TILES>
 __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict__ sc, int ns,
                                              const MatDev* __restrict__ mats, int nmat,
-                                             uint32_t n, SortDev s, GridDev G) {
+                                             uint32_t n, SortDev s, GridDev G, int partials) {
   __shared__ long long tile[TILES * kTileVals];
   __shared__ uint32_t skey[256];
   __shared__ uint32_t sperm[256];
@@ -195,6 +198,11 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
     if (valid) {
       sid = scene_of(sc, ns, j);
       st = make_stencil(sc[sid], P.x[0][p], P.x[1][p], P.x[2][p]);
+    }
+    if (partials && head) {
+      // The workgroup holding a block's first particle records the block's
+      // tile slot; later workgroups of the same block hold it in slot 0.
+      if (s.off[key] + s.tsum[key / kScanTile] == j) s.bslot[key] = (uint32_t)slot;
     }
     if (head && slot < TILES) {
       const SceneDev& T = sc[sid];
@@ -413,9 +421,36 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
   }
   __syncthreads();
 
-  // ---- flush: thread t < 216 owns node t of every tile, one HBM atomic per
-  // non-zero value ----
-  if (t < kTileNodes) {
+  // ---- flush, partial mode: every tile goes to its partial slot with plain
+  // stores, one 8-value record per tile node (thread t < 216 owns node t);
+  // the grid update adds up the partials covering each node ----
+  if (partials && t < kTileNodes) {
+    for (int ti = 0; ti < ntiles; ti++) {
+      const size_t rec = (((size_t)blockIdx.x * TILES + ti) * kTileNodes + t) * kPartStride;
+#if AEMH_DET
+      longlong2* dst = reinterpret_cast<longlong2*>(G.partq + rec);
+#pragma unroll
+      for (int h = 0; h < kPartStride / 2; h++)
+        dst[h] = make_longlong2(tile[ti * kTileVals + (2 * h) * kTileNodes + t],
+                                tile[ti * kTileVals + (2 * h + 1) * kTileNodes + t]);
+#else
+      float v[kPartStride];
+#pragma unroll
+      for (int f = 0; f < kPartStride; f++) {
+        const long long q = f < kTileFields ? tile[ti * kTileVals + f * kTileNodes + t] : 0;
+        const int g = f == 0 ? 0 : (f < 4 ? 1 : 2);
+        v[f] = q == 0 ? 0.0f : (float)((double)q * (double)__int_as_float((127 - 50 + eg[g]) << 23));
+      }
+      float4* dst = reinterpret_cast<float4*>(G.partf + rec);
+      dst[0] = make_float4(v[0], v[1], v[2], v[3]);
+      dst[1] = make_float4(v[4], v[5], v[6], v[7]);
+#endif
+    }
+  }
+  // ---- flush, atomic mode: one HBM atomic per non-zero node value; tile
+  // nodes are enumerated block by block so a wave's lanes share 64-node grid
+  // blocks ----
+  if (!partials && t < kTileNodes) {
     const size_t nodes = G.nodes;
     int lx, ly, lz;
     tile_node_of(t, lx, ly, lz);
@@ -441,9 +476,21 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
   }
 }
 
+// First sorted position of block b's particles (exclusive scan of counts).
+__device__ __forceinline__ uint32_t block_begin(const SortDev& s, uint32_t b, uint32_t nb,
+                                                uint32_t n) {
+  return b < nb ? s.off[b] + s.tsum[b / kScanTile] : n;
+}
+
 // Grid update over the active blocks, one wave per 4^3 block (persistent).
+// A node's P2G sums are its HBM-atomic part plus, in partial-flush mode, the
+// partials of every tile covering it: tiles of the base blocks b - (i, j,
+// k), i, j, k in {0, 1}, reach this block's nodes with local coordinate < 2
+// on each shifted axis, one tile per P2G workgroup holding particles of that
+// base block.  The combined sums are written back for grid readback.
 __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict__ sc, int ns,
-                                                     SortDev s, GridDev G) {
+                                                     SortDev s, GridDev G, int tiles,
+                                                     int partials, uint32_t n, uint32_t nb_total) {
   const uint32_t lane = __lane_id();
   const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * 256u) >> 6;
@@ -458,20 +505,77 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
     const int gx = bx * 4 + (int)(lane & 3), gy = by * 4 + (int)((lane >> 2) & 3),
               gz = bz * 4 + (int)(lane >> 4);
     const uint32_t node = (b << 6) | lane;
+#if AEMH_DET
+    long long sum[kTileFields];
+#pragma unroll
+    for (int f = 0; f < kTileFields; f++) sum[f] = G.accq[(size_t)f * G.nodes + node];
+#else
+    float sum[kTileFields];
+#pragma unroll
+    for (int f = 0; f < kTileFields; f++) sum[f] = G.accf[(size_t)f * G.nodes + node];
+#endif
+    const int lx = (int)(lane & 3), ly = (int)((lane >> 2) & 3), lz = (int)(lane >> 4);
+    // Sorted ranges and first-workgroup tile slots of the 8 candidate base
+    // blocks (wave-uniform), loaded before any partial.
+    uint32_t o0[8], o1[8], sl0[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const int i = c & 1, jj = (c >> 1) & 1, k = c >> 2;
+      o0[c] = o1[c] = 0;
+      sl0[c] = 0xffffffffu;
+      if (!partials || bz < k || by < jj || bx < i) continue;
+      const uint32_t ob = S.block_off + (uint32_t)(((bz - k) * S.nb[1] + (by - jj)) * S.nb[0] + (bx - i));
+      o0[c] = block_begin(s, ob, nb_total, n);
+      o1[c] = block_begin(s, ob + 1, nb_total, n);
+      sl0[c] = s.bslot[ob];
+    }
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      if (o1[c] <= o0[c]) continue;
+      const int i = c & 1, jj = (c >> 1) & 1, k = c >> 2;
+      const bool cover = (i == 0 || lx < 2) && (jj == 0 || ly < 2) && (k == 0 || lz < 2);
+      const int li = ((lz + 4 * k) * kTileSide + (ly + 4 * jj)) * kTileSide + (lx + 4 * i);
+      const uint32_t w0 = o0[c] >> 8, w1 = (o1[c] - 1) >> 8;
+      for (uint32_t w = w0; w <= w1; w++) {
+        const uint32_t sl = (w == w0) ? sl0[c] : 0u;
+        if (sl >= (uint32_t)tiles || !cover) continue;
+        const size_t rec = (((size_t)w * tiles + sl) * kTileNodes + li) * kPartStride;
+#if AEMH_DET
+        const longlong2* src = reinterpret_cast<const longlong2*>(G.partq + rec);
+#pragma unroll
+        for (int h = 0; h < kPartStride / 2; h++) {
+          const longlong2 q = src[h];
+          sum[2 * h] += q.x;
+          sum[2 * h + 1] += q.y;
+        }
+#else
+        const float4* src = reinterpret_cast<const float4*>(G.partf + rec);
+        const float4 a = src[0], b2 = src[1];
+        sum[0] += a.x;
+        sum[1] += a.y;
+        sum[2] += a.z;
+        sum[3] += a.w;
+        sum[4] += b2.x;
+        sum[5] += b2.y;
+        sum[6] += b2.z;
+#endif
+      }
+    }
     float acc[7];
 #pragma unroll
-    for (int f = 0; f < 7; f++) {
+    for (int f = 0; f < kTileFields; f++) {
 #if AEMH_DET
-      acc[f] = fp_to_f32(G.accq[(size_t)f * G.nodes + node]);
+      G.accq[(size_t)f * G.nodes + node] = sum[f];
+      if (f < 7) acc[f] = fp_to_f32(sum[f]);
 #else
-      acc[f] = G.accf[(size_t)f * G.nodes + node];
+      G.accf[(size_t)f * G.nodes + node] = sum[f];
+      acc[f] = sum[f];
 #endif
     }
 #if AEMH_DET
     // sum |c| >= 2^63 raw: some order of the int64 adds could have wrapped
     // (and the saturating FixedPoint sum would depend on order).
-    if ((unsigned long long)G.accq[(size_t)7 * G.nodes + node] >= (1ull << 31))
-      s.counters[kCntOverflow] = 1u;
+    if ((unsigned long long)sum[7] >= (1ull << 31)) s.counters[kCntOverflow] = 1u;
 #endif
     const float m = acc[0];
     float v0 = acc[1], v1 = acc[2], v2 = acc[3];
@@ -744,14 +848,15 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
 }  // namespace
 
 template <int STORE>
-void launch_p2g_store(hipStream_t st, int tiles, unsigned grid, PBuf cur, const SceneDev* sc,
-                      int ns, const MatDev* mats, int nmat, uint32_t n, SortDev s, GridDev g) {
+void launch_p2g_store(hipStream_t st, int tiles, int partials, unsigned grid, PBuf cur,
+                      const SceneDev* sc, int ns, const MatDev* mats, int nmat, uint32_t n,
+                      SortDev s, GridDev g) {
   if (tiles <= 2)
-    k_p2g<STORE, 2><<<grid, 256, 0, st>>>(cur, sc, ns, mats, nmat, n, s, g);
+    k_p2g<STORE, 2><<<grid, 256, 0, st>>>(cur, sc, ns, mats, nmat, n, s, g, partials);
   else if (tiles == 3)
-    k_p2g<STORE, 3><<<grid, 256, 0, st>>>(cur, sc, ns, mats, nmat, n, s, g);
+    k_p2g<STORE, 3><<<grid, 256, 0, st>>>(cur, sc, ns, mats, nmat, n, s, g, partials);
   else
-    k_p2g<STORE, 4><<<grid, 256, 0, st>>>(cur, sc, ns, mats, nmat, n, s, g);
+    k_p2g<STORE, 4><<<grid, 256, 0, st>>>(cur, sc, ns, mats, nmat, n, s, g, partials);
 }
 
 void launch_p2g(hipStream_t st, const LaunchShape& ls, PBuf cur, const SceneDev* sc, int ns,
@@ -759,16 +864,17 @@ void launch_p2g(hipStream_t st, const LaunchShape& ls, PBuf cur, const SceneDev*
   if (!n) return;
   const unsigned grid = (n + 255u) / 256u;
   if (storage == 1)
-    launch_p2g_store<1>(st, ls.p2g_tiles, grid, cur, sc, ns, mats, nmat, n, s, g);
+    launch_p2g_store<1>(st, ls.p2g_tiles, ls.p2g_partials, grid, cur, sc, ns, mats, nmat, n, s, g);
   else if (storage == 2)
-    launch_p2g_store<2>(st, ls.p2g_tiles, grid, cur, sc, ns, mats, nmat, n, s, g);
+    launch_p2g_store<2>(st, ls.p2g_tiles, ls.p2g_partials, grid, cur, sc, ns, mats, nmat, n, s, g);
   else
-    launch_p2g_store<0>(st, ls.p2g_tiles, grid, cur, sc, ns, mats, nmat, n, s, g);
+    launch_p2g_store<0>(st, ls.p2g_tiles, ls.p2g_partials, grid, cur, sc, ns, mats, nmat, n, s, g);
 }
 
 void launch_grid_update(hipStream_t st, const LaunchShape& ls, const SceneDev* sc, int ns,
-                        SortDev s, GridDev g) {
-  k_grid_update<<<ls.upd_wgs, 256, 0, st>>>(sc, ns, s, g);
+                        SortDev s, GridDev g, uint32_t n, uint32_t nb_total) {
+  k_grid_update<<<ls.upd_wgs, 256, 0, st>>>(sc, ns, s, g, ls.p2g_tiles, ls.p2g_partials, n,
+                                            nb_total);
 }
 
 void launch_g2p(hipStream_t st, const LaunchShape& ls, PBuf cur, PBuf nxt, const SceneDev* sc,

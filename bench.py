@@ -215,7 +215,8 @@ def extra_batched(W, aemh, steps=5):
 def probe(W, aemh):
     """Counter-collection run.  The copy moves exactly 256 MiB in and 256 MiB
     out, so FETCH_SIZE / WRITE_SIZE of its dispatch give the KiB each counter
-    reports per real KiB; the P2G / G2P dispatches of the 5 steps follow."""
+    reports per real KiB; the dispatches of the 5 steps after the autotuner
+    (the launch shape the benchmark uses) follow."""
     import torch
     x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
@@ -226,11 +227,11 @@ def probe(W, aemh):
     eng = aemh.Engine()
     W.load_engine(eng, wl)
     eng.step(wl.dt)
-    eng.set_launch_shape(3, 256, 2048)
+    shape = eng.autotune()
     for _ in range(5):
         eng.step(wl.dt)
     eng.close()
-    print("probe done", flush=True)
+    print("probe done, launch shape", shape, flush=True)
 
 
 def main():
@@ -277,7 +278,8 @@ def main():
         "config": {"workload": "c2_snow_1m_128: 1,048,576 snow particles in a dam-break block, "
                                "128^3 grid, dx 1/128, dt 5e-5, FP32 P2G",
                    "particles": wl.n, "grid": [128, 128, 128], "accum": "f32", "substeps": 1,
-                   "launch_shape": {k: shape[k] for k in ("p2g_tiles", "g2p_wg", "upd_wgs")}},
+                   "launch_shape": {k: shape[k] for k in ("p2g_tiles", "p2g_partials", "g2p_wg",
+                                                            "upd_wgs")}},
         "roofline": roofline_of(kern, dominant, npart, blocks, TRAFFIC_C2),
         "step_hbm_frac": round(step_bytes / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},

@@ -144,19 +144,23 @@ aemh_status aemh_step_velocity_only(aemh_ctx* ctx, float total_dt);
  * substep captured in a hipGraph and replayed n_substeps times. */
 aemh_status aemh_set_scenes(aemh_ctx* ctx, const aemh_scene* scenes, uint32_t n);
 aemh_status aemh_batch_step(aemh_ctx* ctx, uint32_t n_substeps);
-/* Times candidate launch shapes on the current particles and scenes (P2G
- * LDS tiles per workgroup 2/3/4, grid-update workgroups 512..4096, G2P
- * workgroup 64/128/256) and keeps the fastest of each; the particle state is
+/* Times candidate launch shapes on the current particles and scenes and
+ * keeps the fastest of each: P2G LDS tiles per workgroup (2/3/4) together
+ * with the tile flush (one HBM atomic per node value, or partial sums stored
+ * per tile and added by the grid update), grid-update workgroups
+ * (1024..16384) and the G2P workgroup (64/128/256).  The particle state is
  * not advanced.  Called once by the first batch step; plain steps use the
  * defaults unless this is called.  No reference counterpart. */
 aemh_status aemh_autotune(aemh_ctx* ctx);
 /* The launch shape in use and, after aemh_autotune, the winning times in ms
- * (P2G, grid update, G2P; zeros before tuning).  NULL outputs are skipped. */
-aemh_status aemh_get_launch_shape(aemh_ctx* ctx, uint32_t* p2g_tiles, uint32_t* g2p_wg,
-                                  uint32_t* upd_wgs, float* tuned_ms3);
-/* Sets the launch shape directly (results do not depend on it). */
-aemh_status aemh_set_launch_shape(aemh_ctx* ctx, uint32_t p2g_tiles, uint32_t g2p_wg,
-                                  uint32_t upd_wgs);
+ * (P2G + grid update, grid update, G2P; zeros before tuning).  NULL outputs
+ * are skipped. */
+aemh_status aemh_get_launch_shape(aemh_ctx* ctx, uint32_t* p2g_tiles, uint32_t* p2g_partials,
+                                  uint32_t* g2p_wg, uint32_t* upd_wgs, float* tuned_ms3);
+/* Sets the launch shape directly (results do not depend on it, bit for bit
+ * in the deterministic mode). */
+aemh_status aemh_set_launch_shape(aemh_ctx* ctx, uint32_t p2g_tiles, uint32_t p2g_partials,
+                                  uint32_t g2p_wg, uint32_t upd_wgs);
 
 /* Grid readback for parity checks (reference layout, x-fastest index
  * i + j nx + k nx ny, mpm_solver.rs:157-159) of scene `scene`, as left by

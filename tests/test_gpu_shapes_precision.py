@@ -56,7 +56,8 @@ def sparse_snow():
     return wl, oracle_run(wl, 10)
 
 
-@pytest.mark.parametrize("shape", [(2, 64, 512), (3, 128, 1024), (4, 256, 4096)])
+@pytest.mark.parametrize("shape", [(2, 0, 64, 1024), (3, 1, 128, 2048), (4, 1, 256, 4096),
+                                   (2, 1, 256, 16384), (4, 0, 128, 8192)])
 def test_launch_shapes_deterministic_bitexact(sparse_snow, shape):
     wl, st = sparse_snow
     eng = aemh.Engine()
@@ -79,8 +80,9 @@ def test_autotune_does_not_advance_state():
         eng.step(wl.dt)
     shape = eng.autotune()
     assert shape["p2g_tiles"] in (2, 3, 4)
+    assert shape["p2g_partials"] in (0, 1)
     assert shape["g2p_wg"] in (64, 128, 256)
-    assert shape["upd_wgs"] in (512, 1024, 2048, 4096)
+    assert shape["upd_wgs"] in (1024, 2048, 4096, 8192, 16384)
     assert all(t > 0 for t in shape["tuned_ms"])
     for _ in range(5):
         eng.step(wl.dt)
@@ -90,10 +92,28 @@ def test_autotune_does_not_advance_state():
 
 def test_launch_shape_rejects_bad_values():
     eng = aemh.Engine()
-    for bad in [(1, 256, 2048), (5, 256, 2048), (3, 96, 2048), (3, 256, 0)]:
+    for bad in [(1, 1, 256, 2048), (5, 1, 256, 2048), (3, 2, 256, 2048), (3, 1, 96, 2048),
+                (3, 1, 256, 0)]:
         with pytest.raises(aemh.AemhError):
             eng.set_launch_shape(*bad)
     eng.close()
+
+
+def test_fp32_flush_modes_agree():
+    """FP32 mode: the atomic and the partial-sum tile flush differ only in
+    the order of the final per-node float additions."""
+    wl = W.c2_snow_dam(n=1 << 16, steps=10, res=64)
+    out = []
+    for part in (0, 1):
+        eng = aemh.Engine()
+        W.load_engine(eng, wl)
+        eng.set_launch_shape(3, part, 256, 2048)
+        for _ in range(10):
+            eng.step(wl.dt)
+        out.append(eng.download())
+        eng.close()
+    for f in ("pos", "vel", "F"):
+        assert rel(out[0][f], out[1][f]) < 1e-6, f
 
 
 def _c5_scaled(storage, n=1 << 16, res=48):
