@@ -606,13 +606,15 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
   }
 }
 
-// Node (v, m) of the block-major grid; callers clamp coords into the grid so
-// every load is in bounds.
-__device__ __forceinline__ float4 load_node(const float4* __restrict__ gv, uint32_t boff, int nb0,
-                                            int nb1, int gx, int gy, int gz) {
-  const uint32_t b = boff + (uint32_t)(((gz >> 2) * nb1 + (gy >> 2)) * nb0 + (gx >> 2));
-  const uint32_t node = (b << 6) | (uint32_t)(((gz & 3) << 4) | ((gy & 3) << 2) | (gx & 3));
-  return gv[node];
+// Block-major node address split by axis: node(x, y, z) = ax + ay + az with
+// ax = (x>>2)<<6 | (x&3), ay = ((y>>2) nb0)<<6 | (y&3)<<2,
+// az = (boff + (z>>2) nb1 nb0)<<6 | (z&3)<<4 -- the in-block fields are
+// disjoint bits below 64, so the sum never carries into the block part.
+__device__ __forceinline__ void node_axes(uint32_t boff, int nb0, int nb1, int x, int y, int z,
+                                          uint32_t& ax, uint32_t& ay, uint32_t& az) {
+  ax = ((uint32_t)(x >> 2) << 6) | (uint32_t)(x & 3);
+  ay = ((uint32_t)((y >> 2) * nb0) << 6) | ((uint32_t)(y & 3) << 2);
+  az = ((boff + (uint32_t)((z >> 2) * nb1 * nb0)) << 6) | ((uint32_t)(z & 3) << 4);
 }
 
 // G2P: sorted position j reads working slot perm[j] of `cur` and writes slot j
@@ -646,6 +648,16 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
     cy[a] = min(st.g[1] + a, n1 - 1);
     cz[a] = min(st.g[2] + a, n2 - 1);
   }
+  // Node addresses: coordinates are clamped into the grid, so every load is
+  // in bounds; skipped nodes are masked after the load.
+  uint32_t ax[3], ay[3], az[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    uint32_t u, v;
+    node_axes(boff, nb0, nb1, cx[a], 0, 0, ax[a], u, v);
+    node_axes(boff, nb0, nb1, 0, cy[a], 0, u, ay[a], v);
+    node_axes(boff, nb0, nb1, 0, 0, cz[a], u, v, az[a]);
+  }
   const float dt = S.dt;
   const uint16_t mi = cur.mat[p];
   nxt.mass[j] = cur.mass[p];
@@ -661,8 +673,9 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
 #pragma unroll
         for (int di = 0; di < 3; di++) {
           const float w0 = wx[di] * (wy[dj] * wz[dk]);
-          const float4 g = load_node(gvp, boff, nb0, nb1, cx[di], cy[dj], cz[dk]);
-          const bool keep = !(w0 < 1e-10f) && ix[di] && iy[dj] && iz[dk] && !(g.w < 1e-12f);
+          const float4 g = gvp[az[dk] + ay[dj] + ax[di]];
+          // bitwise &: every lane loads the whole node (no load behind a branch)
+          const bool keep = !(w0 < 1e-10f) & ix[di] & iy[dj] & iz[dk] & !(g.w < 1e-12f);
           const float w = keep ? w0 : 0.0f;
           nv0 += w * (keep ? g.x : 0.0f);
           nv1 += w * (keep ? g.y : 0.0f);
@@ -736,8 +749,9 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
 #pragma unroll
       for (int di = 0; di < 3; di++) {
         const float w0 = wx[di] * wy[dj] * wz[dk];
-        const float4 g = load_node(gvp, boff, nb0, nb1, cx[di], cy[dj], cz[dk]);
-        const bool keep = !(w0 < 1e-10f) && ix[di] && iy[dj] && iz[dk] && !(g.w < 1e-12f);
+        const float4 g = gvp[az[dk] + ay[dj] + ax[di]];
+        // bitwise &: every lane loads the whole node (no load behind a branch)
+        const bool keep = !(w0 < 1e-10f) & ix[di] & iy[dj] & iz[dk] & !(g.w < 1e-12f);
         const float w = keep ? w0 : 0.0f;
         const float gg[3] = {keep ? g.x : 0.0f, keep ? g.y : 0.0f, keep ? g.z : 0.0f};
         nv[0] += w * gg[0];
