@@ -233,5 +233,19 @@ __device__ __forceinline__ float get_f(const PBuf& P, uint32_t j, int k, int sto
   float d = storage == 1 ? bf16_to_f32(P.Fh[k][j]) : f16_to_f32(P.Fh[k][j]);
   return d + ((k % 4 == 0) ? 1.0f : 0.0f);
 }
+// C and F unchanged from slot p of S to slot j of D, copied in their stored
+// form (no 16-bit round trip through FP32).
+__device__ __forceinline__ void copy_cf(PBuf& D, uint32_t j, const PBuf& S, uint32_t p, int storage) {
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    if (storage == 0) {
+      D.C[k][j] = S.C[k][p];
+      D.F[k][j] = S.F[k][p];
+    } else {
+      D.Ch[k][j] = S.Ch[k][p];
+      D.Fh[k][j] = S.Fh[k][p];
+    }
+  }
+}
 
 }  // namespace aemh

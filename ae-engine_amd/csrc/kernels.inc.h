@@ -629,6 +629,18 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
   const SceneDev S = sc[scene_of(sc, ns, j)];
   const float4* __restrict__ gvp = G.gv;
   const float px = cur.x[0][p], py = cur.x[1][p], pz = cur.x[2][p];
+  // Every other per-particle input is loaded here too, ahead of the first
+  // store to nxt (which the compiler must assume may alias cur), so these
+  // loads overlap the gather instead of following it.
+  const uint16_t mi = cur.mat[p];
+  const float pmass = cur.mass[p];
+  const uint32_t pid = cur.id[p];
+  const float pjp = cur.Jp[p];
+  float F[9];
+  if (!S.vel_only) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) F[k] = get_f(cur, p, k, STORE);
+  }
   const Stencil st = make_stencil(S, px, py, pz);
   const int n0 = S.n[0], n1 = S.n[1], n2 = S.n[2];
   const int nb0 = S.nb[0], nb1 = S.nb[1];
@@ -659,10 +671,9 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
     node_axes(boff, nb0, nb1, 0, 0, cz[a], u, v, az[a]);
   }
   const float dt = S.dt;
-  const uint16_t mi = cur.mat[p];
-  nxt.mass[j] = cur.mass[p];
+  nxt.mass[j] = pmass;
   nxt.mat[j] = mi;
-  nxt.id[j] = cur.id[p];
+  nxt.id[j] = pid;
 
   if (S.vel_only) {  // mpm_solver.rs:1465-1571
     float nv0 = 0.0f, nv1 = 0.0f, nv2 = 0.0f;
@@ -716,13 +727,9 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
     nxt.v[0][j] = nv0 * S.damping;
     nxt.v[1][j] = nv1 * S.damping;
     nxt.v[2][j] = nv2 * S.damping;
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-      put_c(nxt, j, k, get_c(cur, p, k, STORE), STORE);
-      put_f(nxt, j, k, get_f(cur, p, k, STORE), STORE);
-    }
+    copy_cf(nxt, j, cur, p, STORE);
     nxt.J[j] = cur.J[p];
-    nxt.Jp[j] = cur.Jp[p];
+    nxt.Jp[j] = pjp;
     return;
   }
 
@@ -740,6 +747,9 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
   float nv[3] = {0.0f, 0.0f, 0.0f};
   float nc[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   float gv[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#if AEMH_DET
+  // Reference order (mpm_solver.rs:1229-1270): per node, v += w g,
+  // C += (4/dx^2 w) g d^T, grad v -= w g dw^T.
   // One z-layer (9 node loads) in flight: ~133 VGPRs, 3 waves per SIMD.  The
   // fully unrolled 27-node gather needs 256 VGPRs and runs 1.6x slower.
 #pragma unroll 1
@@ -769,6 +779,59 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
 #pragma unroll
           for (int cc = 0; cc < 3; cc++) gv[3 * r + cc] -= w * gg[r] * dw[cc];
       }
+#else
+  // FP32 mode: the same sums regrouped by axis.  With u_r = w g_r per node,
+  // C[r][c] = 4/dx^2 sum_a d_c[a] S_c[a][r] and grad v[r][c] = -sum_a dw_c[a]
+  // S_c[a][r], where S_c[a][r] sums u_r over the nodes whose axis-c offset is
+  // a: 12 operations per node instead of 36, the float rounding that of a
+  // reassociated sum.
+  float sx[3][3], sy[3][3], zc[3] = {0.0f, 0.0f, 0.0f}, zg[3] = {0.0f, 0.0f, 0.0f};
+  float wxy[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      sx[a][r] = sy[a][r] = 0.0f;
+      wxy[a][r] = wx[r] * wy[a];  // [dj][di]
+    }
+#pragma unroll 1
+  for (int dk = 0; dk < 3; dk++) {
+    const float wzk = wz[dk];
+    float t[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int dj = 0; dj < 3; dj++)
+#pragma unroll
+      for (int di = 0; di < 3; di++) {
+        const float w0 = wxy[dj][di] * wzk;
+        const float4 g = gvp[az[dk] + ay[dj] + ax[di]];
+        // bitwise &: every lane loads the whole node (no load behind a branch)
+        const bool keep = !(w0 < 1e-10f) & ix[di] & iy[dj] & iz[dk] & !(g.w < 1e-12f);
+        const float u[3] = {keep ? w0 * g.x : 0.0f, keep ? w0 * g.y : 0.0f, keep ? w0 * g.z : 0.0f};
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          sx[di][r] += u[r];
+          sy[dj][r] += u[r];
+          t[r] += u[r];
+        }
+      }
+    const float dzk = dza[dk], dwzk = dwz[dk];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      nv[r] += t[r];
+      zc[r] += t[r] * dzk;
+      zg[r] += t[r] * dwzk;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    nc[3 * r + 0] = k4 * (sx[0][r] * dxa[0] + sx[1][r] * dxa[1] + sx[2][r] * dxa[2]);
+    nc[3 * r + 1] = k4 * (sy[0][r] * dya[0] + sy[1][r] * dya[1] + sy[2][r] * dya[2]);
+    nc[3 * r + 2] = k4 * zc[r];
+    gv[3 * r + 0] = -(sx[0][r] * dwx[0] + sx[1][r] * dwx[1] + sx[2][r] * dwx[2]);
+    gv[3 * r + 1] = -(sy[0][r] * dwy[0] + sy[1][r] * dwy[1] + sy[2][r] * dwy[2]);
+    gv[3 * r + 2] = -zg[r];
+  }
+#endif
   // Q7: non-finite velocity zeroes v and C (grad v kept); +-50 clamp.
   if (!isfinite(nv[0]) || !isfinite(nv[1]) || !isfinite(nv[2])) {
     nv[0] = nv[1] = nv[2] = 0.0f;
@@ -821,9 +884,7 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
 #pragma unroll
   for (int k = 0; k < 9; k++) put_c(nxt, j, k, nc[k], STORE);
   // Q9: F' = (I + dt grad v) F; validity; J = clamp(det F', 0.1, 10).
-  float F[9], nf[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) F[k] = get_f(cur, p, k, STORE);
+  float nf[9];
 #pragma unroll
   for (int r = 0; r < 3; r++)
 #pragma unroll
@@ -841,7 +902,7 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
 #pragma unroll
   for (int k = 0; k < 9; k++)
     if (!isfinite(nf[k]) || fabsf(nf[k]) > 1e3f) f_valid = false;
-  float jp = cur.Jp[p];
+  float jp = pjp;
   float fj;
   if (f_valid) {
     const MatDev& M = mats[mi < (uint32_t)nmat ? mi : (uint32_t)nmat];
