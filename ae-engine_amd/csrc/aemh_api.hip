@@ -247,6 +247,30 @@ SceneDev make_scene(const aemh_scene& s, const aemh_config& cfg, uint32_t block_
   return d;
 }
 
+// P2G tile partials (partial-flush mode only, allocated on first use): room
+// for the largest tile budget, 216 node records of 8 values per tile, per
+// 256-particle workgroup.
+aemh_status ensure_partials(aemh_ctx* c) {
+  const int q32 = c->cfg.accum == AEMH_ACCUM_Q32_32;
+  const size_t wgs = ((size_t)(c->n_work ? c->n_work : 1) + 255) / 256;
+  if (wgs <= c->part_wg_cap && q32 == c->part_q32 && (c->grid.partf || c->grid.partq))
+    return AEMH_OK;
+  if (c->grid.partf) hipFree(c->grid.partf);
+  if (c->grid.partq) hipFree(c->grid.partq);
+  c->grid.partf = nullptr;
+  c->grid.partq = nullptr;
+  c->part_wg_cap = 0;
+  const size_t vals = wgs * kMaxTileBudget * (size_t)kTileNodes * kDetFields;
+  if (q32)
+    HIPTRY(c, hipMalloc(&c->grid.partq, vals * sizeof(long long)));
+  else
+    HIPTRY(c, hipMalloc(&c->grid.partf, vals * sizeof(float)));
+  c->part_wg_cap = wgs;
+  c->part_q32 = q32;
+  invalidate_graphs(c);
+  return AEMH_OK;
+}
+
 aemh_status ensure_grid_sort(aemh_ctx* c) {
   const uint32_t nb = c->nb_total;
   const uint32_t n = c->n_work;
@@ -296,25 +320,6 @@ aemh_status ensure_grid_sort(aemh_ctx* c) {
   if (!c->sort.counters) {
     HIPTRY(c, hipMalloc(&c->sort.counters, 64));
     HIPTRY(c, hipMemset(c->sort.counters, 0, 64));
-  }
-  // P2G tile partials: room for the largest tile budget, 216 nodes x fields
-  // per tile, one slot per 256-particle workgroup.
-  {
-    const size_t wgs = ((size_t)(n ? n : 1) + 255) / 256;
-    const size_t vals = wgs * kMaxTileBudget * (size_t)kDetFields * kTileNodes;
-    if (wgs > c->part_wg_cap || q32 != c->part_q32 || (!c->grid.partf && !c->grid.partq)) {
-      if (c->grid.partf) hipFree(c->grid.partf);
-      if (c->grid.partq) hipFree(c->grid.partq);
-      c->grid.partf = nullptr;
-      c->grid.partq = nullptr;
-      if (q32)
-        HIPTRY(c, hipMalloc(&c->grid.partq, vals * sizeof(long long)));
-      else
-        HIPTRY(c, hipMalloc(&c->grid.partf, vals * sizeof(float)));
-      c->part_wg_cap = wgs;
-      c->part_q32 = q32;
-      invalidate_graphs(c);
-    }
   }
   if (n > c->sort_n_cap || !c->sort.key) {
     for (void* p : {(void*)c->sort.key, (void*)c->sort.rank, (void*)c->sort.perm})
@@ -377,8 +382,10 @@ hipEvent_t get_event(aemh_ctx* c) {
     c->ev_pool.pop_back();
     return e;
   }
+  // No system-scope fence: a fenced record also writes back L2, which is
+  // not needed for a timestamp on this stream.
   hipEvent_t e = nullptr;
-  hipEventCreate(&e);
+  hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
   return e;
 }
 
@@ -492,6 +499,13 @@ aemh_status check_overflow(aemh_ctx* c) {
 aemh_status run_substeps(aemh_ctx* c, uint32_t k) {
   if (!c->mats_d) return fail(c, AEMH_ERR_STATE, "materials not set");
   if (c->n_work == 0) return AEMH_OK;
+  if (c->shape.p2g_partials) {
+    aemh_status s = ensure_partials(c);
+    if (s != AEMH_OK) return s;
+  }
+  // Steps replay the captured substep graph.  With profiling on they launch
+  // eagerly with an event pair around each kernel section: event-record
+  // nodes inside a graph cost the replay more than eager launches do.
   const bool graph = c->use_graph && !c->prof;
   if (graph && !c->graph_valid) {
     aemh_status s = build_graphs(c);
@@ -919,6 +933,8 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   int pick = ls.p2g_tiles, pick_part = ls.p2g_partials;
   // Tile budget and flush mode together: the flush mode moves work between
   // P2G and the grid update, so both are timed.
+  s = ensure_partials(c);
+  if (s != AEMH_OK) return s;
   for (int part = 0; part <= 1; part++)
     for (int tiles = 2; tiles <= 4; tiles++) {
       ls.p2g_tiles = tiles;

@@ -13,8 +13,8 @@ namespace aemh {
 // workload's best).  P2G always runs 256 particles per workgroup.
 struct LaunchShape {
   int p2g_tiles = 3;     // LDS tiles per P2G workgroup: 2, 3 or 4
-  int p2g_partials = 1;  // P2G tile flush: 1 partial stores gathered by the
-                         // grid update, 0 one HBM atomic per node value
+  int p2g_partials = 0;  // P2G tile flush: 0 one HBM atomic per node value,
+                         // 1 partial stores gathered by the grid update
   int g2p_wg = 256;      // G2P workgroup size: 64, 128 or 256
   int upd_wgs = 2048;    // persistent grid-update / clear workgroups
 };

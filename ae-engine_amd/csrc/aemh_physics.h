@@ -108,9 +108,18 @@ __device__ __forceinline__ void neo_hookean(const float* F, float J, float mu, f
   }
 }
 
+// 3x3 matrix by value: the out-of-line models take and return it in
+// registers, so callers need no stack copy of F or the stress.
+struct Mat3 {
+  float m[9];
+};
+
 // Drucker-Prager / von Mises, constitutive.rs:443-601 (stress only; the
 // corrected F is discarded by P2G, mpm_solver.rs:843-844).
-__device__ __noinline__ void plastic_stress(const MatDev& M, const float* F, float* st) {
+__device__ __noinline__ Mat3 plastic_stress(const MatDev M, const Mat3 Fm) {
+  const float* F = Fm.m;
+  Mat3 out;
+  float* st = out.m;
   float r[9], s[9];
   polar_decomp(F, r, s);
   const float mu = M.mu, lambda = M.lambda;
@@ -147,9 +156,13 @@ __device__ __noinline__ void plastic_stress(const MatDev& M, const float* F, flo
   st[0] -= pressure;
   st[4] -= pressure;
   st[8] -= pressure;
+  return out;
 }
 
-__device__ __noinline__ void corotated_stress(const MatDev& M, const float* F, float J, float* st) {
+__device__ __noinline__ Mat3 corotated_stress(const MatDev M, const Mat3 Fm, float J) {
+  const float* F = Fm.m;
+  Mat3 out;
+  float* st = out.m;
   float r[9], s[9];
   polar_decomp(F, r, s);
   if (M.model == kModelCorotated) {  // constitutive.rs:395-415
@@ -170,6 +183,7 @@ __device__ __noinline__ void corotated_stress(const MatDev& M, const float* F, f
     st[4] += M.lambda * (J - 1.0f) * J;
     st[8] += M.lambda * (J - 1.0f) * J;
   }
+  return out;
 }
 
 // compute_stress dispatch, constitutive.rs:234-249, plus the snow model
@@ -191,13 +205,25 @@ __device__ __forceinline__ void compute_stress(const MatDev& M, const float* F, 
       break;
     }
     case kModelCorotated:
-    case kModelFixedCorotated:
-      corotated_stress(M, F, J, st);
+    case kModelFixedCorotated: {
+      Mat3 Fm;
+#pragma unroll
+      for (int i = 0; i < 9; i++) Fm.m[i] = F[i];
+      const Mat3 o = corotated_stress(M, Fm, J);
+#pragma unroll
+      for (int i = 0; i < 9; i++) st[i] = o.m[i];
       break;
+    }
     case kModelDruckerPrager:
-    case kModelVonMises:
-      plastic_stress(M, F, st);
+    case kModelVonMises: {
+      Mat3 Fm;
+#pragma unroll
+      for (int i = 0; i < 9; i++) Fm.m[i] = F[i];
+      const Mat3 o = plastic_stress(M, Fm);
+#pragma unroll
+      for (int i = 0; i < 9; i++) st[i] = o.m[i];
       break;
+    }
     case kModelSnow: {
       float h = m_exp(M.xi * (1.0f - Jp));
       h = fminf(fmaxf(h, 0.1f), 5.0f);

@@ -180,7 +180,9 @@ aemh_status aemh_conservation_f64(aemh_ctx* ctx, double* out5);
 aemh_status aemh_last_substep_info(aemh_ctx* ctx, uint32_t* active_blocks,
                                    uint32_t* particles);
 
-/* Per-kernel HIP-event timing on the context stream (eager launches). */
+/* Per-kernel timing: while enabled, substeps launch eagerly with a HIP event
+ * pair around each kernel section on the context stream (instead of the
+ * graph replay); aemh_profile_get gives the average ms per section. */
 aemh_status aemh_profile_enable(aemh_ctx* ctx, int on);
 aemh_status aemh_profile_get(aemh_ctx* ctx, const char* kernel, double* avg_ms,
                              uint64_t* count);

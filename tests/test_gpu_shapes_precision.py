@@ -90,6 +90,28 @@ def test_autotune_does_not_advance_state():
     eng.close()
 
 
+def test_profiled_steps_are_exact_and_timed():
+    """Profiled steps launch eagerly with events around each section: results
+    stay bit-exact with the graph replays and every section gets a time."""
+    wl = W.c1_elastic_cube()
+    wl.accum = aemh.ACCUM_Q32_32
+    st = oracle_run(wl, 6)
+    eng = aemh.Engine()
+    W.load_engine(eng, wl)
+    eng.step(wl.dt)
+    eng.profile(True)
+    eng.profile_reset()
+    for _ in range(4):
+        eng.step(wl.dt)
+    times = {k: eng.profile_get(k) for k in ("count", "scan", "p2g", "grid_update", "g2p")}
+    eng.profile(False)
+    eng.step(wl.dt)
+    assert_bitexact(eng.download(), st)
+    for k, (ms, n) in times.items():
+        assert n == 4 and 0.0 < ms < 50.0, (k, ms, n)
+    eng.close()
+
+
 def test_launch_shape_rejects_bad_values():
     eng = aemh.Engine()
     for bad in [(1, 1, 256, 2048), (5, 1, 256, 2048), (3, 2, 256, 2048), (3, 1, 96, 2048),
