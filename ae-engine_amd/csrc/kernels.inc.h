@@ -40,6 +40,8 @@ constexpr int kTileVals = kTileFields * kTileNodes;
 constexpr int kPartStride = 8;
 // Lane layers of the P2G re-assignment (rank within a base cell, capped).
 constexpr int kLayers = 16;
+// Base-block tiles of grid (v, m) a G2P workgroup stages in LDS.
+constexpr int kG2PTiles = 4;
 
 // Per-particle P2G inputs after the constitutive update.
 struct P2GPart {
@@ -623,10 +625,67 @@ template <int STORE>
 __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev* __restrict__ sc,
                                              int ns, const MatDev* __restrict__ mats, int nmat,
                                              uint32_t n, SortDev s, GridDev G) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const uint32_t p = s.perm[j];
-  const SceneDev S = sc[scene_of(sc, ns, j)];
+  // The grid nodes of the workgroup's first kG2PTiles base blocks are staged
+  // in LDS (one 6^3 tile each, loaded in one coalesced pass), so the 27-node
+  // gather reads LDS instead of waiting on three dependent rounds of global
+  // loads; particles of further blocks gather from global memory.
+  __shared__ float4 gtile[kG2PTiles * kTileNodes];
+  __shared__ uint32_t skey[256];
+  __shared__ int gmeta[kG2PTiles][8];  // origin x, y, z; grid n x, y, z; nb0; nb1
+  __shared__ uint32_t gboff[kG2PTiles];
+  __shared__ uint32_t wcnt[4];
+  const int t = threadIdx.x;
+  const int lane = __lane_id(), wave = t >> 6;
+  const uint32_t j = blockIdx.x * blockDim.x + (uint32_t)t;
+  const bool valid = j < n;
+  uint32_t p = 0, key = 0xffffffffu;
+  if (valid) {
+    p = s.perm[j];
+    key = s.key[p];
+  }
+  skey[t] = key;
+  __syncthreads();
+  const bool head = valid && (t == 0 || skey[t - 1] != key);
+  const unsigned long long hb = __ballot(head);
+  const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  if (lane == 0) wcnt[wave] = (uint32_t)__popcll(hb);
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+    if (w < wave) pre += wcnt[w];
+    tot += wcnt[w];
+  }
+  const int slot = (int)(pre + (uint32_t)__popcll(hb & le)) - 1;
+  const int ntiles = (int)min(tot, (uint32_t)kG2PTiles);
+  const SceneDev S = sc[scene_of(sc, ns, valid ? j : 0u)];
+  if (head && slot < kG2PTiles) {
+    const uint32_t lb = key - S.block_off;
+    gmeta[slot][0] = (int)(lb % (uint32_t)S.nb[0]) * 4;
+    gmeta[slot][1] = (int)((lb / (uint32_t)S.nb[0]) % (uint32_t)S.nb[1]) * 4;
+    gmeta[slot][2] = (int)(lb / (uint32_t)(S.nb[0] * S.nb[1])) * 4;
+    gmeta[slot][3] = S.n[0];
+    gmeta[slot][4] = S.n[1];
+    gmeta[slot][5] = S.n[2];
+    gmeta[slot][6] = S.nb[0];
+    gmeta[slot][7] = S.nb[1];
+    gboff[slot] = S.block_off;
+  }
+  __syncthreads();
+  for (int e = t; e < ntiles * kTileNodes; e += (int)blockDim.x) {
+    const int ti = e / kTileNodes, li = e - ti * kTileNodes;
+    const int x = gmeta[ti][0] + li % kTileSide, y = gmeta[ti][1] + (li / kTileSide) % kTileSide,
+              z = gmeta[ti][2] + li / (kTileSide * kTileSide);
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // outside the grid: never kept
+    if (x < gmeta[ti][3] && y < gmeta[ti][4] && z < gmeta[ti][5]) {
+      uint32_t ux, uy, uz;
+      node_axes(gboff[ti], gmeta[ti][6], gmeta[ti][7], x, y, z, ux, uy, uz);
+      v = G.gv[ux + uy + uz];
+    }
+    gtile[e] = v;
+  }
+  __syncthreads();
+  if (!valid) return;
+  const bool use_lds = slot < kG2PTiles;
   const float4* __restrict__ gvp = G.gv;
   const float px = cur.x[0][p], py = cur.x[1][p], pz = cur.x[2][p];
   // Every other per-particle input is loaded here too, ahead of the first
@@ -670,6 +729,21 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
     node_axes(boff, nb0, nb1, 0, cy[a], 0, u, ay[a], v);
     node_axes(boff, nb0, nb1, 0, 0, cz[a], u, v, az[a]);
   }
+  // LDS tile offsets (clamped coordinates lie in the base block's 6^3 halo)
+  int tx[3] = {0, 0, 0}, ty[3] = {0, 0, 0}, tz[3] = {0, 0, 0};
+  if (use_lds) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      tx[a] = slot * kTileNodes + (cx[a] - gmeta[slot][0]);
+      ty[a] = (cy[a] - gmeta[slot][1]) * kTileSide;
+      tz[a] = (cz[a] - gmeta[slot][2]) * (kTileSide * kTileSide);
+    }
+  }
+  // Node (v, m) from the LDS tile or from the grid.  Each gather loop below
+  // is instantiated once per source (a select between the two pointers
+  // would turn every load into a generic flat load).
+  auto lds_node = [&](int di, int dj, int dk) -> float4 { return gtile[tz[dk] + ty[dj] + tx[di]]; };
+  auto hbm_node = [&](int di, int dj, int dk) -> float4 { return gvp[az[dk] + ay[dj] + ax[di]]; };
   const float dt = S.dt;
   nxt.mass[j] = pmass;
   nxt.mat[j] = mi;
@@ -677,21 +751,27 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
 
   if (S.vel_only) {  // mpm_solver.rs:1465-1571
     float nv0 = 0.0f, nv1 = 0.0f, nv2 = 0.0f;
+    auto gather = [&](auto node_at) {
 #pragma unroll
-    for (int dk = 0; dk < 3; dk++)
+      for (int dk = 0; dk < 3; dk++)
 #pragma unroll
-      for (int dj = 0; dj < 3; dj++)
+        for (int dj = 0; dj < 3; dj++)
 #pragma unroll
-        for (int di = 0; di < 3; di++) {
-          const float w0 = wx[di] * (wy[dj] * wz[dk]);
-          const float4 g = gvp[az[dk] + ay[dj] + ax[di]];
-          // bitwise &: every lane loads the whole node (no load behind a branch)
-          const bool keep = !(w0 < 1e-10f) & ix[di] & iy[dj] & iz[dk] & !(g.w < 1e-12f);
-          const float w = keep ? w0 : 0.0f;
-          nv0 += w * (keep ? g.x : 0.0f);
-          nv1 += w * (keep ? g.y : 0.0f);
-          nv2 += w * (keep ? g.z : 0.0f);
-        }
+          for (int di = 0; di < 3; di++) {
+            const float w0 = wx[di] * (wy[dj] * wz[dk]);
+            const float4 g = node_at(di, dj, dk);
+            // bitwise &: every lane loads the whole node (no load behind a branch)
+            const bool keep = !(w0 < 1e-10f) & ix[di] & iy[dj] & iz[dk] & !(g.w < 1e-12f);
+            const float w = keep ? w0 : 0.0f;
+            nv0 += w * (keep ? g.x : 0.0f);
+            nv1 += w * (keep ? g.y : 0.0f);
+            nv2 += w * (keep ? g.z : 0.0f);
+          }
+    };
+    if (use_lds)
+      gather(lds_node);
+    else
+      gather(hbm_node);
     if (!isfinite(nv0) || !isfinite(nv1) || !isfinite(nv2)) nv0 = nv1 = nv2 = 0.0f;
     nv0 = fminf(fmaxf(nv0, -50.0f), 50.0f);
     nv1 = fminf(fmaxf(nv1, -50.0f), 50.0f);
@@ -752,6 +832,7 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
   // C += (4/dx^2 w) g d^T, grad v -= w g dw^T.
   // One z-layer (9 node loads) in flight: ~133 VGPRs, 3 waves per SIMD.  The
   // fully unrolled 27-node gather needs 256 VGPRs and runs 1.6x slower.
+  auto gather = [&](auto node_at) {
 #pragma unroll 1
   for (int dk = 0; dk < 3; dk++)
 #pragma unroll
@@ -759,7 +840,7 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
 #pragma unroll
       for (int di = 0; di < 3; di++) {
         const float w0 = wx[di] * wy[dj] * wz[dk];
-        const float4 g = gvp[az[dk] + ay[dj] + ax[di]];
+        const float4 g = node_at(di, dj, dk);
         // bitwise &: every lane loads the whole node (no load behind a branch)
         const bool keep = !(w0 < 1e-10f) & ix[di] & iy[dj] & iz[dk] & !(g.w < 1e-12f);
         const float w = keep ? w0 : 0.0f;
@@ -779,6 +860,11 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
 #pragma unroll
           for (int cc = 0; cc < 3; cc++) gv[3 * r + cc] -= w * gg[r] * dw[cc];
       }
+  };
+  if (use_lds)
+    gather(lds_node);
+  else
+    gather(hbm_node);
 #else
   // FP32 mode: the same sums regrouped by axis.  With u_r = w g_r per node,
   // C[r][c] = 4/dx^2 sum_a d_c[a] S_c[a][r] and grad v[r][c] = -sum_a dw_c[a]
@@ -794,6 +880,7 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
       sx[a][r] = sy[a][r] = 0.0f;
       wxy[a][r] = wx[r] * wy[a];  // [dj][di]
     }
+  auto gather = [&](auto node_at) {
 #pragma unroll 1
   for (int dk = 0; dk < 3; dk++) {
     const float wzk = wz[dk];
@@ -803,7 +890,7 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
 #pragma unroll
       for (int di = 0; di < 3; di++) {
         const float w0 = wxy[dj][di] * wzk;
-        const float4 g = gvp[az[dk] + ay[dj] + ax[di]];
+        const float4 g = node_at(di, dj, dk);
         // bitwise &: every lane loads the whole node (no load behind a branch)
         const bool keep = !(w0 < 1e-10f) & ix[di] & iy[dj] & iz[dk] & !(g.w < 1e-12f);
         const float u[3] = {keep ? w0 * g.x : 0.0f, keep ? w0 * g.y : 0.0f, keep ? w0 * g.z : 0.0f};
@@ -822,6 +909,11 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
       zg[r] += t[r] * dwzk;
     }
   }
+  };
+  if (use_lds)
+    gather(lds_node);
+  else
+    gather(hbm_node);
 #pragma unroll
   for (int r = 0; r < 3; r++) {
     nc[3 * r + 0] = k4 * (sx[0][r] * dxa[0] + sx[1][r] * dxa[1] + sx[2][r] * dxa[2]);
