@@ -44,7 +44,7 @@ KERNELS = ["step_begin", "count", "scan", "scatter", "clear", "p2g", "grid_updat
 # HBM bytes per dispatch on the 1M-particle snow workload, from rocprofv3
 # FETCH_SIZE (x 2.000) and WRITE_SIZE (x 1.000) against a 256 MiB copy, as
 # recorded in profiles/c2_snow_1m_hbm_traffic.md for this build's kernels.
-TRAFFIC_C2 = {"p2g": 408.2e6, "g2p": 224.5e6, "grid_update": 26.5e6}
+TRAFFIC_C2 = {"p2g": 246.9e6, "g2p": 229.8e6, "grid_update": 140.1e6}
 
 
 def log(msg):
