@@ -76,7 +76,11 @@ EXPORTS = [
     "aemh_download_grid", "aemh_download_grid_q32", "aemh_conservation_f64",
     "aemh_last_substep_info", "aemh_profile_enable", "aemh_profile_get", "aemh_profile_reset",
     "aemh_scene_uniform_box",
+    "aemh_mpss_reserve", "aemh_mpss_spawn", "aemh_mpss_kill", "aemh_mpss_compact",
+    "aemh_mpss_update_lifetimes", "aemh_mpss_write", "aemh_mpss_read", "aemh_mpss_state",
 ]
+
+NO_SLOT = 0xFFFFFFFF  # spawn() on a full buffer (None in the reference)
 
 _lib = None
 
@@ -125,6 +129,16 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.aemh_profile_reset.argtypes = [P]
     lib.aemh_scene_uniform_box.argtypes = [C.c_uint64, u32, fp, fp, fp]
     lib.aemh_scene_uniform_box.restype = None
+    pu32 = C.POINTER(u32)
+    pu16 = C.POINTER(C.c_uint16)
+    lib.aemh_mpss_reserve.argtypes = [P, u32]
+    lib.aemh_mpss_spawn.argtypes = [P, u32, pu32, pu32]
+    lib.aemh_mpss_kill.argtypes = [P, pu32, u32]
+    lib.aemh_mpss_compact.argtypes = [P]
+    lib.aemh_mpss_update_lifetimes.argtypes = [P, C.c_float, pu32, u32, pu32]
+    lib.aemh_mpss_write.argtypes = [P, pu32, u32, fp, fp, fp, fp, fp, fp, fp, pu16, fp, fp]
+    lib.aemh_mpss_read.argtypes = [P, pu32, u32, fp, fp, fp, fp, fp, fp, fp, pu16, fp, fp]
+    lib.aemh_mpss_state.argtypes = [P, pu32, pu32, C.POINTER(C.c_uint8)]
     for name in EXPORTS:
         f = getattr(lib, name)
         if name not in ("aemh_version", "aemh_last_error", "aemh_config_default",
@@ -140,6 +154,21 @@ def _f(a):
 
 def _c(a, dtype):
     return None if a is None else np.ascontiguousarray(a, dtype=dtype)
+
+
+def _u32(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _u16(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_uint16))
+
+
+# Per-slot fields of the resident buffer: name -> (components, dtype)
+MPSS_FIELDS = {"pos": (3, np.float32), "vel": (3, np.float32), "C": (9, np.float32),
+               "F": (9, np.float32), "J": (1, np.float32), "Jp": (1, np.float32),
+               "mass": (1, np.float32), "mat": (1, np.uint16), "lifetime": (1, np.float32),
+               "age": (1, np.float32)}
 
 
 def default_config() -> Config:
@@ -332,3 +361,123 @@ class Engine:
         cnt = C.c_uint64()
         self._check(self._lib.aemh_profile_get(self._h, kernel.encode(), C.byref(ms), C.byref(cnt)))
         return ms.value, cnt.value
+
+    # -- resident MpssBuffer (mpss.rs:161-297, 431-445) ---------------------------
+    def mpss_reserve(self, capacity: int):
+        self._check(self._lib.aemh_mpss_reserve(self._h, capacity))
+        self.n = capacity
+
+    def mpss_spawn(self, n: int) -> np.ndarray:
+        """spawn() n times; slots, NO_SLOT where the buffer was full."""
+        out = np.empty(max(n, 1), np.uint32)
+        got = C.c_uint32(0)
+        self._check(self._lib.aemh_mpss_spawn(self._h, n, _u32(out), C.byref(got)))
+        return out[:n]
+
+    def mpss_kill(self, slots):
+        slots = _c(np.atleast_1d(slots), np.uint32)
+        self._check(self._lib.aemh_mpss_kill(self._h, _u32(slots), slots.size))
+
+    def mpss_compact(self):
+        self._check(self._lib.aemh_mpss_compact(self._h))
+
+    def mpss_update_lifetimes(self, dt: float) -> np.ndarray:
+        out = np.empty(max(self.n, 1), np.uint32)
+        ne = C.c_uint32(0)
+        self._check(self._lib.aemh_mpss_update_lifetimes(self._h, C.c_float(dt), _u32(out),
+                                                         out.size, C.byref(ne)))
+        return out[:ne.value].copy()
+
+    def mpss_write(self, slots, **fields):
+        slots = _c(np.atleast_1d(slots), np.uint32)
+        n = slots.size
+        args = []
+        for name, (comps, dt) in MPSS_FIELDS.items():
+            a = fields.pop(name, None)
+            if a is not None:
+                a = _c(a, dt).reshape(-1)
+                if a.size != n * comps:
+                    raise ValueError(f"{name}: expected {n * comps} values, got {a.size}")
+            args.append(a)
+        if fields:
+            raise ValueError(f"unknown fields {sorted(fields)}")
+        ptrs = [_u16(a) if name == "mat" else _f(a) for name, a in zip(MPSS_FIELDS, args)]
+        self._check(self._lib.aemh_mpss_write(self._h, _u32(slots), n, *ptrs))
+
+    def mpss_read(self, slots, fields=tuple(MPSS_FIELDS)):
+        slots = _c(np.atleast_1d(slots), np.uint32)
+        n = slots.size
+        out = {}
+        ptrs = []
+        for name, (comps, dt) in MPSS_FIELDS.items():
+            if name in fields:
+                a = np.empty((n, comps) if comps > 1 else n, dt)
+                out[name] = a
+                ptrs.append(_u16(a) if name == "mat" else _f(a))
+            else:
+                ptrs.append(None)
+        self._check(self._lib.aemh_mpss_read(self._h, _u32(slots), n, *ptrs))
+        return out
+
+    def mpss_state(self):
+        """(count, capacity, active flags as bool[capacity])."""
+        cnt = C.c_uint32(0)
+        cap = C.c_uint32(0)
+        self._check(self._lib.aemh_mpss_state(self._h, C.byref(cnt), C.byref(cap), None))
+        act = np.empty(cap.value, np.uint8)
+        self._check(self._lib.aemh_mpss_state(self._h, None, None,
+                                              act.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return cnt.value, cap.value, act.astype(bool)
+
+
+class MpssBuffer:
+    """The reference's MpssBuffer interface (ae_particle/src/mpss.rs) over an
+    Engine whose particle arrays stay in HBM.
+
+    spawn() returns a slot or None, kill(idx) ignores dead or out-of-range
+    slots, compact() and update_lifetimes(dt) follow mpss.rs:283-297 and
+    :431-445; field access goes through read()/write() by slot list.
+    """
+
+    def __init__(self, engine: Engine, capacity: int):
+        self.engine = engine
+        self.capacity = capacity
+        engine.mpss_reserve(capacity)
+
+    def spawn(self):
+        s = int(self.engine.mpss_spawn(1)[0])
+        return None if s == NO_SLOT else s
+
+    def spawn_many(self, n: int) -> np.ndarray:
+        return self.engine.mpss_spawn(n)
+
+    def kill(self, idx):
+        self.engine.mpss_kill(idx)
+
+    def compact(self):
+        self.engine.mpss_compact()
+
+    def update_lifetimes(self, dt: float):
+        return [int(i) for i in self.engine.mpss_update_lifetimes(dt)]
+
+    def len(self) -> int:
+        return self.engine.mpss_state()[0]
+
+    def __len__(self) -> int:
+        return self.len()
+
+    def is_empty(self) -> bool:
+        return self.len() == 0
+
+    @property
+    def active(self) -> np.ndarray:
+        return self.engine.mpss_state()[2]
+
+    def active_indices(self) -> np.ndarray:
+        return np.flatnonzero(self.active).astype(np.uint32)
+
+    def write(self, slots, **fields):
+        self.engine.mpss_write(slots, **fields)
+
+    def read(self, slots, fields=tuple(MPSS_FIELDS)):
+        return self.engine.mpss_read(slots, fields)

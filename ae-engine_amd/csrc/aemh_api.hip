@@ -13,6 +13,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <float.h>
+
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <string>
@@ -65,6 +68,20 @@ struct aemh_ctx {
   uint32_t* idx_d = nullptr;
   float* staging = nullptr;
   size_t staging_bytes = 0;
+  // resident MpssBuffer lifecycle (aemh_mpss_reserve): device lifecycle
+  // arrays of the canonical store, host bookkeeping mirroring mpss.rs
+  bool mpss = false;
+  MpssDev life{};
+  MpssDev life_tmp{};
+  ParticleStore canon_tmp;
+  std::vector<uint8_t> act_h;
+  std::vector<uint32_t> free_list;
+  uint32_t mpss_count = 0;
+  uint32_t* slots_d = nullptr;  // slot lists / permutation for lifecycle kernels
+  size_t slots_cap = 0;
+  uint32_t* expired_d = nullptr;  // [1 + cap]: count, then expired slots
+  bool idx_explicit = false;      // the caller fixed the list (aemh_set_indices)
+  bool work_default = false;      // the work set is the default set below
   // scenes
   std::vector<aemh_scene> scenes_h;  // batch table as given
   bool batch_mode = false;
@@ -431,6 +448,75 @@ aemh_status time_launches(aemh_ctx* c, int reps, float* ms, Fn&& fn) {
   return AEMH_OK;
 }
 
+aemh_status flush_work(aemh_ctx* c);
+
+// Device copy of a slot list (or permutation) for the lifecycle kernels.
+aemh_status upload_slots(aemh_ctx* c, const uint32_t* slots, size_t n) {
+  if (n > c->slots_cap) {
+    if (c->slots_d) hipFree(c->slots_d);
+    c->slots_d = nullptr;
+    c->slots_cap = 0;
+    HIPTRY(c, hipMalloc(&c->slots_d, n * 4));
+    c->slots_cap = n;
+  }
+  if (n) HIPTRY(c, hipMemcpyAsync(c->slots_d, slots, n * 4, hipMemcpyHostToDevice, c->stream));
+  return AEMH_OK;
+}
+
+aemh_status alloc_life(aemh_ctx* c, MpssDev& L, uint32_t cap) {
+  for (void* p : {(void*)L.lifetime, (void*)L.age, (void*)L.active})
+    if (p) hipFree(p);
+  L = MpssDev{};
+  HIPTRY(c, hipMalloc(&L.lifetime, (size_t)cap * 4));
+  HIPTRY(c, hipMalloc(&L.age, (size_t)cap * 4));
+  HIPTRY(c, hipMalloc(&L.active, (size_t)cap));
+  return AEMH_OK;
+}
+
+// MpssBuffer::kill for each slot in order (mpss.rs:205-212): an active slot
+// becomes inactive and goes on the free list.
+aemh_status mpss_kill(aemh_ctx* c, const uint32_t* slots, uint32_t n) {
+  std::vector<uint32_t> killed;
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t k = slots[i];
+    if (k < c->canon.cap && c->act_h[k]) {
+      c->act_h[k] = 0;
+      c->free_list.push_back(k);
+      c->mpss_count -= 1;
+      killed.push_back(k);
+    }
+  }
+  if (killed.empty()) return AEMH_OK;
+  aemh_status s = upload_slots(c, killed.data(), killed.size());
+  if (s != AEMH_OK) return s;
+  launch_mpss_set_active(c->stream, c->life.active, c->slots_d, (uint32_t)killed.size(), 0);
+  CHECK_LAUNCH(c);
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  // The default work set (active slots below count) changed; an explicit
+  // index list stays as the caller gave it.
+  if (!c->idx_explicit) {
+    if ((s = flush_work(c)) != AEMH_OK) return s;
+    c->work_valid = false;
+  }
+  return AEMH_OK;
+}
+
+// Before the canonical slots are rewritten: the work buffers go back first
+// and are gathered again at the next step.
+aemh_status mpss_begin_write(aemh_ctx* c) {
+  aemh_status s = flush_work(c);
+  if (s != AEMH_OK) return s;
+  c->work_valid = false;
+  return AEMH_OK;
+}
+
+aemh_status mpss_ready(aemh_ctx* c) {
+  if (!c) return AEMH_ERR_ARG;
+  if (!c->mpss) return fail(c, AEMH_ERR_STATE, "no resident MpssBuffer (aemh_mpss_reserve first)");
+  hipSetDevice(c->device);
+  return AEMH_OK;
+}
+
 // Launch one substep on the context stream (eager or under capture).
 void enqueue_substep(aemh_ctx* c, int cur) {
   const PBuf a = c->work[cur].pb, b = c->work[cur ^ 1].pb;
@@ -527,11 +613,23 @@ aemh_status run_substeps(aemh_ctx* c, uint32_t k) {
 }
 
 aemh_status ensure_work(aemh_ctx* c) {
-  if (c->work_valid) return AEMH_OK;
+  if (c->work_valid && (c->idx_explicit || c->work_default || !c->mpss)) return AEMH_OK;
   if (!c->canon.slab) return fail(c, AEMH_ERR_STATE, "no particles uploaded");
-  std::vector<uint32_t> all(c->canon.cap);
-  for (uint32_t i = 0; i < c->canon.cap; i++) all[i] = i;
-  return aemh_set_indices(c, all.data(), c->canon.cap);
+  if (c->idx_explicit) {
+    const std::vector<uint32_t> keep = c->idx_h;
+    return aemh_set_indices(c, keep.data(), (uint32_t)keep.size());
+  }
+  // No list given: every slot; for a resident MpssBuffer the slots the serial
+  // substep visits, i < count and active, ascending (mpm_solver.rs:187-193).
+  std::vector<uint32_t> all;
+  all.reserve(c->canon.cap);
+  const uint32_t lim = c->mpss ? c->mpss_count : c->canon.cap;
+  for (uint32_t i = 0; i < lim; i++)
+    if (!c->mpss || c->act_h[i]) all.push_back(i);
+  aemh_status s = aemh_set_indices(c, all.data(), (uint32_t)all.size());
+  c->idx_explicit = false;
+  c->work_default = s == AEMH_OK;
+  return s;
 }
 
 aemh_status flush_work(aemh_ctx* c) {
@@ -566,15 +664,15 @@ aemh_status prepare_single_scene(aemh_ctx* c, float sub_dt, int vel_only) {
 
 aemh_status single_scene(aemh_ctx* c, float total_dt, int vel_only) {
   if (c->cfg.substeps == 0) return fail(c, AEMH_ERR_ARG, "substeps must be >= 1");
-  aemh_status s = prepare_single_scene(c, total_dt / (float)c->cfg.substeps,  // mpm_solver.rs:1075
+  if (!c->grid_set) return fail(c, AEMH_ERR_STATE, "grid not set");
+  aemh_status s = ensure_work(c);
+  if (s != AEMH_OK) return s;
+  if (c->n_work == 0) return AEMH_OK;  // nothing to advance
+  s = prepare_single_scene(c, total_dt / (float)c->cfg.substeps,  // mpm_solver.rs:1075
                                        vel_only);
   if (s != AEMH_OK) return s;
   return run_substeps(c, c->cfg.substeps);
 }
-
-struct Ptr9 {
-  float* p[9];
-};
 
 }  // namespace
 
@@ -675,7 +773,10 @@ aemh_status aemh_destroy(aemh_ctx* c) {
                   (void*)c->grid.accq, (void*)c->grid.gv, (void*)c->sort.key, (void*)c->sort.rank,
                   (void*)c->sort.perm, (void*)c->sort.cnt, (void*)c->sort.off, (void*)c->sort.tsum,
                   (void*)c->sort.flag, (void*)c->sort.active, (void*)c->sort.counters,
-                  (void*)c->sort.bslot, (void*)c->grid.partf, (void*)c->grid.partq})
+                  (void*)c->sort.bslot, (void*)c->grid.partf, (void*)c->grid.partq,
+                  (void*)c->life.lifetime, (void*)c->life.age, (void*)c->life.active,
+                  (void*)c->life_tmp.lifetime, (void*)c->life_tmp.age, (void*)c->life_tmp.active,
+                  c->canon_tmp.slab, (void*)c->slots_d, (void*)c->expired_d})
     if (p) hipFree(p);
   if (c->flag_h) hipHostFree(c->flag_h);
   hipStreamDestroy(c->stream);
@@ -773,6 +874,8 @@ aemh_status aemh_upload_particles(aemh_ctx* c, uint32_t n, const float* pos3, co
   CHECK_LAUNCH(c);
   c->work_valid = false;
   c->work_dirty = false;
+  c->idx_explicit = false;
+  c->mpss = false;
   if (c->sort.counters) HIPTRY(c, hipMemset(c->sort.counters, 0, 64));
   return AEMH_OK;
 }
@@ -839,6 +942,8 @@ aemh_status aemh_set_indices(aemh_ctx* c, const uint32_t* idx, uint32_t n_idx) {
   HIPTRY(c, hipMalloc(&c->idx_d, (size_t)(n_idx ? n_idx : 1) * 4));
   if (n_idx) HIPTRY(c, hipMemcpy(c->idx_d, idx, (size_t)n_idx * 4, hipMemcpyHostToDevice));
   c->idx_h.assign(idx, idx + n_idx);
+  c->idx_explicit = true;
+  c->work_default = false;
   c->n_work = n_idx;
   c->cur = 0;
   launch_gather_work(c->stream, c->canon.pb, c->work[0].pb, c->idx_d, n_idx, storage);
@@ -854,20 +959,32 @@ aemh_status aemh_set_indices(aemh_ctx* c, const uint32_t* idx, uint32_t n_idx) {
 aemh_status aemh_step(aemh_ctx* c, float total_dt) {
   if (!c) return AEMH_ERR_ARG;
   hipSetDevice(c->device);
-  return single_scene(c, total_dt, 0);
+  aemh_status s = single_scene(c, total_dt, 0);
+  if (s != AEMH_OK || !c->mpss) return s;
+  uint32_t n_exp = 0;  // mpm_step_with_grid ends with update_lifetimes (:508)
+  return aemh_mpss_update_lifetimes(c, total_dt, nullptr, 0, &n_exp);
 }
 
 aemh_status aemh_mpm_step_parallel_with_grid(aemh_ctx* c, const uint32_t* idx, uint32_t n_idx,
                                              float total_dt) {
   if (!c) return AEMH_ERR_ARG;
   if (n_idx == 0) return AEMH_OK;  // mpm_solver.rs:1072-1074
+  if (!idx) return fail(c, AEMH_ERR_ARG, "null index list");
   hipSetDevice(c->device);
   if (!c->work_valid || c->idx_h.size() != n_idx ||
       memcmp(c->idx_h.data(), idx, (size_t)n_idx * 4) != 0) {
     aemh_status s = aemh_set_indices(c, idx, n_idx);
     if (s != AEMH_OK) return s;
   }
-  return single_scene(c, total_dt, 0);
+  c->idx_explicit = true;
+  aemh_status s = single_scene(c, total_dt, 0);
+  // With a resident buffer the list holds for this call only, as the
+  // reference takes particle_indices per call.
+  if (c->mpss) c->idx_explicit = false;
+  if (s != AEMH_OK || !c->mpss) return s;
+  // buffer.update_lifetimes(total_dt) closes the reference step (:1083).
+  uint32_t n_exp = 0;
+  return aemh_mpss_update_lifetimes(c, total_dt, nullptr, 0, &n_exp);
 }
 
 aemh_status aemh_step_velocity_only(aemh_ctx* c, float total_dt) {
@@ -1123,6 +1240,248 @@ aemh_status aemh_profile_get(aemh_ctx* c, const char* kernel, double* avg_ms, ui
 aemh_status aemh_profile_reset(aemh_ctx* c) {
   if (!c) return AEMH_ERR_ARG;
   c->prof_rec.clear();
+  return AEMH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Resident MpssBuffer lifecycle (mpss.rs:161-297, 431-445)
+// ---------------------------------------------------------------------------
+aemh_status aemh_mpss_reserve(aemh_ctx* c, uint32_t capacity) {
+  if (!c) return AEMH_ERR_ARG;
+  if (capacity == 0) return fail(c, AEMH_ERR_ARG, "capacity must be > 0");
+  hipSetDevice(c->device);
+  aemh_status s = alloc_store(c, c->canon, capacity, 0);
+  if (s != AEMH_OK) return s;
+  c->canon.cap = capacity;
+  if ((s = alloc_life(c, c->life, capacity)) != AEMH_OK) return s;
+  // compaction targets are sized at the first compact() of this capacity
+  for (void* p : {(void*)c->life_tmp.lifetime, (void*)c->life_tmp.age,
+                  (void*)c->life_tmp.active, c->canon_tmp.slab})
+    if (p) hipFree(p);
+  c->life_tmp = MpssDev{};
+  c->canon_tmp = ParticleStore{};
+  PBuf& P = c->canon.pb;
+  // MpssBuffer::new (mpss.rs:161-187): strain and C zero, jacobian and mass
+  // 1, lifetime f32::MAX, age 0, material 0, nothing active.
+  for (int a = 0; a < 3; a++) {
+    launch_fill(c->stream, P.x[a], 0.0f, capacity);
+    launch_fill(c->stream, P.v[a], 0.0f, capacity);
+  }
+  for (int k = 0; k < 9; k++) {
+    launch_fill(c->stream, P.C[k], 0.0f, capacity);
+    launch_fill(c->stream, P.F[k], 0.0f, capacity);
+  }
+  launch_fill(c->stream, P.J, 1.0f, capacity);
+  launch_fill(c->stream, P.Jp, 1.0f, capacity);
+  launch_fill(c->stream, P.mass, 1.0f, capacity);
+  launch_fill(c->stream, c->life.lifetime, FLT_MAX, capacity);
+  launch_fill(c->stream, c->life.age, 0.0f, capacity);
+  HIPTRY(c, hipMemsetAsync(P.mat, 0, (size_t)capacity * 2, c->stream));
+  HIPTRY(c, hipMemsetAsync(c->life.active, 0, capacity, c->stream));
+  CHECK_LAUNCH(c);
+  if (c->expired_d) hipFree(c->expired_d);
+  c->expired_d = nullptr;
+  HIPTRY(c, hipMalloc(&c->expired_d, ((size_t)capacity + 1) * 4));
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  c->act_h.assign(capacity, 0);
+  c->free_list.clear();
+  c->mpss_count = 0;
+  c->mpss = true;
+  c->work_valid = false;
+  c->work_dirty = false;
+  c->idx_explicit = false;
+  c->idx_h.clear();
+  if (c->sort.counters) HIPTRY(c, hipMemset(c->sort.counters, 0, 64));
+  return AEMH_OK;
+}
+
+aemh_status aemh_mpss_spawn(aemh_ctx* c, uint32_t n, uint32_t* slots_out, uint32_t* n_spawned) {
+  aemh_status s = mpss_ready(c);
+  if (s != AEMH_OK) return s;
+  if (n && !slots_out) return fail(c, AEMH_ERR_ARG, "null slot output");
+  // spawn() n times (mpss.rs:189-203): the free list is popped last-in
+  // first-out, then fresh slots at `count`; a full buffer yields None.
+  std::vector<uint32_t> got;
+  got.reserve(n);
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t idx = AEMH_NO_SLOT;
+    if (!c->free_list.empty()) {
+      idx = c->free_list.back();
+      c->free_list.pop_back();
+    } else if (c->mpss_count < c->canon.cap) {
+      idx = c->mpss_count;
+    }
+    slots_out[i] = idx;
+    if (idx == AEMH_NO_SLOT) continue;
+    c->act_h[idx] = 1;
+    c->mpss_count += 1;
+    got.push_back(idx);
+  }
+  if (n_spawned) *n_spawned = (uint32_t)got.size();
+  if (got.empty()) return AEMH_OK;
+  if ((s = mpss_begin_write(c)) != AEMH_OK) return s;
+  if ((s = upload_slots(c, got.data(), got.size())) != AEMH_OK) return s;
+  launch_mpss_reset(c->stream, c->canon.pb, c->life, c->slots_d, (uint32_t)got.size());
+  CHECK_LAUNCH(c);
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  return AEMH_OK;
+}
+
+aemh_status aemh_mpss_kill(aemh_ctx* c, const uint32_t* slots, uint32_t n) {
+  aemh_status s = mpss_ready(c);
+  if (s != AEMH_OK) return s;
+  if (n && !slots) return fail(c, AEMH_ERR_ARG, "null slot list");
+  return mpss_kill(c, slots, n);
+}
+
+aemh_status aemh_mpss_compact(aemh_ctx* c) {
+  aemh_status s = mpss_ready(c);
+  if (s != AEMH_OK) return s;
+  const uint32_t cap = c->canon.cap;
+  // compact() (mpss.rs:283-297): the reference's swap sequence, replayed on
+  // slot labels, gives the permutation the device gather applies at once.
+  std::vector<uint32_t> perm(cap);
+  for (uint32_t i = 0; i < cap; i++) perm[i] = i;
+  uint32_t write = 0;
+  for (uint32_t read = 0; read < cap; read++) {
+    if (!c->act_h[perm[read]]) continue;
+    if (write != read) std::swap(perm[write], perm[read]);
+    write++;
+  }
+  if ((s = mpss_begin_write(c)) != AEMH_OK) return s;
+  if ((s = alloc_store(c, c->canon_tmp, cap, 0)) != AEMH_OK) return s;
+  if (!c->life_tmp.lifetime && (s = alloc_life(c, c->life_tmp, cap)) != AEMH_OK) return s;
+  if ((s = upload_slots(c, perm.data(), cap)) != AEMH_OK) return s;
+  launch_mpss_gather(c->stream, c->canon.pb, c->life, c->canon_tmp.pb, c->life_tmp, c->slots_d, cap,
+                     write);
+  CHECK_LAUNCH(c);
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  std::swap(c->canon, c->canon_tmp);
+  std::swap(c->life, c->life_tmp);
+  c->canon.cap = cap;
+  c->mpss_count = write;
+  c->free_list.clear();
+  for (uint32_t i = 0; i < cap; i++) c->act_h[i] = i < write ? 1 : 0;
+  return AEMH_OK;
+}
+
+aemh_status aemh_mpss_update_lifetimes(aemh_ctx* c, float dt, uint32_t* expired_out,
+                                       uint32_t max_out, uint32_t* n_expired) {
+  aemh_status s = mpss_ready(c);
+  if (s != AEMH_OK) return s;
+  if (n_expired) *n_expired = 0;
+  if (c->mpss_count == 0) return AEMH_OK;
+  // update_lifetimes (mpss.rs:431-445): active slots below count age by dt;
+  // those at or past their lifetime are killed in ascending slot order.
+  HIPTRY(c, hipMemsetAsync(c->expired_d, 0, 4, c->stream));
+  launch_mpss_age(c->stream, c->life, c->mpss_count, dt, c->expired_d + 1, c->expired_d);
+  CHECK_LAUNCH(c);
+  uint32_t ne = 0;
+  HIPTRY(c, hipMemcpyAsync(&ne, c->expired_d, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  if (n_expired) *n_expired = ne;
+  if (ne == 0) return AEMH_OK;
+  std::vector<uint32_t> ex(ne);
+  HIPTRY(c, hipMemcpy(ex.data(), c->expired_d + 1, (size_t)ne * 4, hipMemcpyDeviceToHost));
+  std::sort(ex.begin(), ex.end());
+  if (expired_out) memcpy(expired_out, ex.data(), (size_t)std::min(ne, max_out) * 4);
+  return mpss_kill(c, ex.data(), ne);
+}
+
+aemh_status aemh_mpss_write(aemh_ctx* c, const uint32_t* slots, uint32_t n, const float* pos3,
+                            const float* vel3, const float* C9, const float* F9, const float* J,
+                            const float* Jp, const float* mass, const uint16_t* mat,
+                            const float* lifetime, const float* age) {
+  aemh_status s = mpss_ready(c);
+  if (s != AEMH_OK) return s;
+  if (n == 0) return AEMH_OK;
+  if (!slots) return fail(c, AEMH_ERR_ARG, "null slot list");
+  for (uint32_t i = 0; i < n; i++)
+    if (slots[i] >= c->canon.cap)
+      return fail(c, AEMH_ERR_ARG, "slot %u out of range (capacity %u)", slots[i], c->canon.cap);
+  if ((s = mpss_begin_write(c)) != AEMH_OK) return s;
+  if ((s = upload_slots(c, slots, n)) != AEMH_OK) return s;
+  PBuf& P = c->canon.pb;
+  struct Field {
+    const float* h;
+    int comps;
+    float* const* d;
+  };
+  float* one[3] = {P.J, P.Jp, P.mass};
+  const Field fields[] = {{pos3, 3, P.x},      {vel3, 3, P.v},     {C9, 9, P.C},
+                          {F9, 9, P.F},        {J, 1, &one[0]},    {Jp, 1, &one[1]},
+                          {mass, 1, &one[2]},  {lifetime, 1, &c->life.lifetime},
+                          {age, 1, &c->life.age}};
+  for (const Field& f : fields) {
+    if (!f.h) continue;
+    if ((s = ensure_staging(c, (size_t)n * f.comps * 4)) != AEMH_OK) return s;
+    HIPTRY(c, hipMemcpyAsync(c->staging, f.h, (size_t)n * f.comps * 4, hipMemcpyHostToDevice,
+                             c->stream));
+    Ptr9 d{};
+    for (int k = 0; k < f.comps; k++) d.p[k] = f.d[k];
+    launch_rows_to_slots(c->stream, c->staging, d, f.comps, c->slots_d, n);
+    CHECK_LAUNCH(c);
+  }
+  if (mat) {
+    if ((s = ensure_staging(c, (size_t)n * 2)) != AEMH_OK) return s;
+    HIPTRY(c, hipMemcpyAsync(c->staging, mat, (size_t)n * 2, hipMemcpyHostToDevice, c->stream));
+    launch_u16_to_slots(c->stream, (const uint16_t*)c->staging, P.mat, c->slots_d, n);
+    CHECK_LAUNCH(c);
+  }
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  return AEMH_OK;
+}
+
+aemh_status aemh_mpss_read(aemh_ctx* c, const uint32_t* slots, uint32_t n, float* pos3, float* vel3,
+                           float* C9, float* F9, float* J, float* Jp, float* mass, uint16_t* mat,
+                           float* lifetime, float* age) {
+  aemh_status s = mpss_ready(c);
+  if (s != AEMH_OK) return s;
+  if (n == 0) return AEMH_OK;
+  if (!slots) return fail(c, AEMH_ERR_ARG, "null slot list");
+  for (uint32_t i = 0; i < n; i++)
+    if (slots[i] >= c->canon.cap)
+      return fail(c, AEMH_ERR_ARG, "slot %u out of range (capacity %u)", slots[i], c->canon.cap);
+  if ((s = flush_work(c)) != AEMH_OK) return s;
+  if ((s = upload_slots(c, slots, n)) != AEMH_OK) return s;
+  PBuf& P = c->canon.pb;
+  struct Field {
+    float* h;
+    int comps;
+    float* const* d;
+  };
+  float* one[3] = {P.J, P.Jp, P.mass};
+  const Field fields[] = {{pos3, 3, P.x},      {vel3, 3, P.v},     {C9, 9, P.C},
+                          {F9, 9, P.F},        {J, 1, &one[0]},    {Jp, 1, &one[1]},
+                          {mass, 1, &one[2]},  {lifetime, 1, &c->life.lifetime},
+                          {age, 1, &c->life.age}};
+  for (const Field& f : fields) {
+    if (!f.h) continue;
+    if ((s = ensure_staging(c, (size_t)n * f.comps * 4)) != AEMH_OK) return s;
+    Ptr9 d{};
+    for (int k = 0; k < f.comps; k++) d.p[k] = f.d[k];
+    launch_slots_to_rows(c->stream, d, c->staging, f.comps, c->slots_d, n);
+    CHECK_LAUNCH(c);
+    HIPTRY(c, hipMemcpyAsync(f.h, c->staging, (size_t)n * f.comps * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+  }
+  if (mat) {
+    if ((s = ensure_staging(c, (size_t)n * 2)) != AEMH_OK) return s;
+    launch_slots_to_u16(c->stream, P.mat, (uint16_t*)c->staging, c->slots_d, n);
+    CHECK_LAUNCH(c);
+    HIPTRY(c, hipMemcpyAsync(mat, c->staging, (size_t)n * 2, hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+  }
+  return AEMH_OK;
+}
+
+aemh_status aemh_mpss_state(aemh_ctx* c, uint32_t* count, uint32_t* capacity, uint8_t* active) {
+  aemh_status s = mpss_ready(c);
+  if (s != AEMH_OK) return s;
+  if (count) *count = c->mpss_count;
+  if (capacity) *capacity = c->canon.cap;
+  if (active) memcpy(active, c->act_h.data(), c->canon.cap);
   return AEMH_OK;
 }
 

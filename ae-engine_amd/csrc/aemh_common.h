@@ -69,6 +69,18 @@ struct PBuf {
   uint32_t* id;     // position in the active index list
 };
 
+// Lifecycle fields of a resident MpssBuffer (mpss.rs:104-157).
+struct MpssDev {
+  float* lifetime;
+  float* age;
+  uint8_t* active;
+};
+
+// Up to 9 component arrays of one SoA field.
+struct Ptr9 {
+  float* p[9];
+};
+
 // Grid accumulators and the updated velocity read by G2P.
 struct GridDev {
   float* accf;      // [7][nodes] m, p0..2, f0..2 (FP32 mode)

@@ -39,6 +39,23 @@ void launch_grid_readback(hipStream_t st, const SceneDev* sc_d, int scene, Scene
 // FP64 sums over the working set: mass, momentum xyz, kinetic energy.
 void launch_conservation(hipStream_t st, PBuf cur, uint32_t n, double* out5);
 
+// ---- k_mpss.hip: resident MpssBuffer lifecycle ------------------------------
+void launch_mpss_reset(hipStream_t st, PBuf P, MpssDev L, const uint32_t* slots, uint32_t n);
+void launch_mpss_set_active(hipStream_t st, uint8_t* act, const uint32_t* slots, uint32_t n,
+                            uint8_t v);
+void launch_mpss_gather(hipStream_t st, PBuf S, MpssDev LS, PBuf D, MpssDev LD,
+                        const uint32_t* perm, uint32_t cap, uint32_t count);
+void launch_mpss_age(hipStream_t st, MpssDev L, uint32_t count, float dt, uint32_t* out,
+                     uint32_t* nout);
+void launch_rows_to_slots(hipStream_t st, const float* rows, Ptr9 dst, int comps,
+                          const uint32_t* slots, uint32_t n);
+void launch_slots_to_rows(hipStream_t st, Ptr9 src, float* rows, int comps, const uint32_t* slots,
+                          uint32_t n);
+void launch_u16_to_slots(hipStream_t st, const uint16_t* v, uint16_t* dst, const uint32_t* slots,
+                         uint32_t n);
+void launch_slots_to_u16(hipStream_t st, const uint16_t* src, uint16_t* v, const uint32_t* slots,
+                         uint32_t n);
+
 // ---- k_fast.hip (FP32 mode) and k_det.hip (deterministic mode) -----------
 #define AEMH_DECLARE_MODE(NS)                                                               \
   namespace NS {                                                                            \

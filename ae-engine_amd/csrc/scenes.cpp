@@ -1,6 +1,7 @@
 // scenes.cpp -- deterministic synthetic particle generators (host only).
 // xoshiro256** (Blackman & Vigna) seeded through splitmix64, so a workload
 // is fixed by its seed on every machine.
+#include <stddef.h>
 #include <stdint.h>
 
 #include "../../include/aemh.h"

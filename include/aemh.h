@@ -129,6 +129,54 @@ aemh_status aemh_download_plastic_j(aemh_ctx* ctx, float* Jp);
 /* The active set: particle_indices of mpm_step_parallel_with_grid. */
 aemh_status aemh_set_indices(aemh_ctx* ctx, const uint32_t* idx, uint32_t n_idx);
 
+/* Resident MpssBuffer (ae_particle/src/mpss.rs:104-297, 431-445) ------- */
+/* Slot value of a spawn that found the buffer full (spawn() -> None). */
+#define AEMH_NO_SLOT 0xFFFFFFFFu
+/* MpssBuffer::new(capacity) (mpss.rs:161-187), resident in HBM: every slot
+ * inactive with the new() defaults (strain and C zero, jacobian and mass 1,
+ * lifetime f32::MAX, age 0, material 0).  Replaces any uploaded particles.
+ * While a buffer is resident, aemh_step and aemh_mpm_step_parallel_with_grid
+ * end with update_lifetimes(total_dt) as the reference steps do
+ * (mpm_solver.rs:489, :508, :1083), and aemh_step without an index list
+ * advances the active slots below count, the serial substep's set
+ * (mpm_solver.rs:187-193). */
+aemh_status aemh_mpss_reserve(aemh_ctx* ctx, uint32_t capacity);
+/* spawn() n times (mpss.rs:189-203): free-list slots last-in first-out, then
+ * fresh slots at count; each is reset (reset_slot, mpss.rs:215-232: pos and
+ * vel 0, strain identity, jacobian 1, mass 1, material 0, lifetime
+ * f32::MAX, age 0; C keeps its contents) and made active.  slots_out[n]
+ * gets the slot or AEMH_NO_SLOT; *n_spawned (may be NULL) the number made. */
+aemh_status aemh_mpss_spawn(aemh_ctx* ctx, uint32_t n, uint32_t* slots_out,
+                            uint32_t* n_spawned);
+/* kill() for each slot in order (mpss.rs:205-212); inactive or out-of-range
+ * slots are ignored as the reference ignores them. */
+aemh_status aemh_mpss_kill(aemh_ctx* ctx, const uint32_t* slots, uint32_t n);
+/* compact() (mpss.rs:283-297): the active slots move to the front in slot
+ * order with every field; the vacated slots hold what the reference's swap
+ * sequence leaves there; the free list is cleared. */
+aemh_status aemh_mpss_compact(aemh_ctx* ctx);
+/* update_lifetimes(dt) (mpss.rs:431-445): active slots below count age by
+ * dt; those with age >= lifetime are killed in ascending slot order.  The
+ * expired slots (ascending) go to expired_out, at most max_out of them
+ * (NULL: not returned); *n_expired gets their total. */
+aemh_status aemh_mpss_update_lifetimes(aemh_ctx* ctx, float dt, uint32_t* expired_out,
+                                       uint32_t max_out, uint32_t* n_expired);
+/* Writes / reads the fields of n given slots (rows in slot-list order; NULL
+ * fields are skipped): the pub field access of MpssBuffer (mpss.rs:104-157),
+ * plus J_p of the snow model. */
+aemh_status aemh_mpss_write(aemh_ctx* ctx, const uint32_t* slots, uint32_t n,
+                            const float* pos3, const float* vel3, const float* C9,
+                            const float* F9, const float* J, const float* Jp,
+                            const float* mass, const uint16_t* mat,
+                            const float* lifetime, const float* age);
+aemh_status aemh_mpss_read(aemh_ctx* ctx, const uint32_t* slots, uint32_t n,
+                           float* pos3, float* vel3, float* C9, float* F9, float* J,
+                           float* Jp, float* mass, uint16_t* mat, float* lifetime,
+                           float* age);
+/* len() / capacity and the active flags (capacity bytes; NULL: skipped). */
+aemh_status aemh_mpss_state(aemh_ctx* ctx, uint32_t* count, uint32_t* capacity,
+                            uint8_t* active);
+
 /* Steps ----------------------------------------------------------------- */
 /* mpm_step_parallel_with_grid (mpm_solver.rs:1063-1084): substeps of
  * total_dt/substeps over the active set on the context grid. */
