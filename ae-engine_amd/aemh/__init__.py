@@ -35,6 +35,37 @@ class AemhError(RuntimeError):
         self.status = status
 
 
+class LodConfig(C.Structure):
+    """aemh_lod_config: PhysicsLod (simulation.rs:27-48) + tier grids."""
+    _fields_ = [
+        ("near_field_distance", C.c_float),
+        ("mid_field_distance", C.c_float),
+        ("far_field_update_interval", C.c_uint32),
+        ("max_near_particles", C.c_uint32),
+        ("mid_interval", C.c_uint32),
+        ("near_n", C.c_uint32),
+        ("near_dx", C.c_float),
+        ("mid_n", C.c_uint32),
+        ("mid_dx", C.c_float),
+        ("reserved", C.c_uint32 * 3),
+    ]
+
+
+class TierStats(C.Structure):
+    _fields_ = [
+        ("near", C.c_uint32), ("mid", C.c_uint32), ("far", C.c_uint32),
+        ("reclassified", C.c_uint32), ("near_ran", C.c_uint32), ("mid_ran", C.c_uint32),
+        ("far_ran", C.c_uint32), ("expired", C.c_uint32),
+        ("near_origin", C.c_float * 3), ("mid_origin", C.c_float * 3),
+    ]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["near_origin"] = tuple(self.near_origin)
+        d["mid_origin"] = tuple(self.mid_origin)
+        return d
+
+
 class Config(C.Structure):
     _fields_ = [
         ("substeps", C.c_uint32),
@@ -78,6 +109,8 @@ EXPORTS = [
     "aemh_scene_uniform_box",
     "aemh_mpss_reserve", "aemh_mpss_spawn", "aemh_mpss_kill", "aemh_mpss_compact",
     "aemh_mpss_update_lifetimes", "aemh_mpss_write", "aemh_mpss_read", "aemh_mpss_state",
+    "aemh_lod_config_default", "aemh_tier_set_config", "aemh_tier_tick", "aemh_tier_lists",
+    "aemh_tier_render",
 ]
 
 NO_SLOT = 0xFFFFFFFF  # spawn() on a full buffer (None in the reference)
@@ -139,10 +172,16 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.aemh_mpss_write.argtypes = [P, pu32, u32, fp, fp, fp, fp, fp, fp, fp, pu16, fp, fp]
     lib.aemh_mpss_read.argtypes = [P, pu32, u32, fp, fp, fp, fp, fp, fp, fp, pu16, fp, fp]
     lib.aemh_mpss_state.argtypes = [P, pu32, pu32, C.POINTER(C.c_uint8)]
+    lib.aemh_lod_config_default.argtypes = [C.POINTER(LodConfig)]
+    lib.aemh_lod_config_default.restype = None
+    lib.aemh_tier_set_config.argtypes = [P, C.POINTER(LodConfig)]
+    lib.aemh_tier_tick.argtypes = [P, fp, C.c_float, C.POINTER(TierStats)]
+    lib.aemh_tier_lists.argtypes = [P, pu32, pu32, pu32, pu32]
+    lib.aemh_tier_render.argtypes = [P, u32, u32, fp, pu32, pu32]
     for name in EXPORTS:
         f = getattr(lib, name)
         if name not in ("aemh_version", "aemh_last_error", "aemh_config_default",
-                        "aemh_scene_uniform_box"):
+                        "aemh_scene_uniform_box", "aemh_lod_config_default"):
             f.restype = C.c_int
     _lib = lib
     return lib
@@ -169,6 +208,12 @@ MPSS_FIELDS = {"pos": (3, np.float32), "vel": (3, np.float32), "C": (9, np.float
                "F": (9, np.float32), "J": (1, np.float32), "Jp": (1, np.float32),
                "mass": (1, np.float32), "mat": (1, np.uint16), "lifetime": (1, np.float32),
                "age": (1, np.float32)}
+
+
+def default_lod_config() -> LodConfig:
+    cfg = LodConfig()
+    load_library().aemh_lod_config_default(C.byref(cfg))
+    return cfg
 
 
 def default_config() -> Config:
@@ -428,6 +473,35 @@ class Engine:
         self._check(self._lib.aemh_mpss_state(self._h, None, None,
                                               act.ctypes.data_as(C.POINTER(C.c_uint8))))
         return cnt.value, cap.value, act.astype(bool)
+
+    # -- LOD tier orchestrator (simulation.rs:480-700) ----------------------------
+    def tier_set_config(self, lod: LodConfig):
+        self._check(self._lib.aemh_tier_set_config(self._h, C.byref(lod)))
+
+    def tier_tick(self, player, dt: float) -> dict:
+        p = np.ascontiguousarray(player, np.float32).reshape(3)
+        st = TierStats()
+        self._check(self._lib.aemh_tier_tick(self._h, _f(p), C.c_float(dt), C.byref(st)))
+        return st.as_dict()
+
+    def tier_lists(self):
+        """(near, mid, far) cached slot lists."""
+        n3 = np.zeros(3, np.uint32)
+        self._check(self._lib.aemh_tier_lists(self._h, None, None, None, _u32(n3)))
+        ls = [np.empty(max(int(k), 1), np.uint32) for k in n3]
+        self._check(self._lib.aemh_tier_lists(self._h, _u32(ls[0]), _u32(ls[1]), _u32(ls[2]),
+                                              None))
+        return tuple(a[:int(k)] for a, k in zip(ls, n3))
+
+    def tier_render(self, tier: int, take: int):
+        """Live entries among the first `take` of a cached list: (pos[n,3], slots[n])."""
+        m = max(take, 1)
+        pos = np.empty((m, 3), np.float32)
+        slots = np.empty(m, np.uint32)
+        n = C.c_uint32(0)
+        self._check(self._lib.aemh_tier_render(self._h, tier, take, _f(pos), _u32(slots),
+                                               C.byref(n)))
+        return pos[:n.value].copy(), slots[:n.value].copy()
 
 
 class MpssBuffer:

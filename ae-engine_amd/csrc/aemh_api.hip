@@ -82,6 +82,25 @@ struct aemh_ctx {
   uint32_t* expired_d = nullptr;  // [1 + cap]: count, then expired slots
   bool idx_explicit = false;      // the caller fixed the list (aemh_set_indices)
   bool work_default = false;      // the work set is the default set below
+  size_t idx_cap = 0;
+  // LOD tier orchestrator (aemh_tier_tick): counters and cached lists of
+  // SimulationManager (simulation.rs:97-103, 480-560), lists on device
+  aemh_lod_config lod{};
+  bool lod_set = false;
+  uint32_t field_counter = 0, lod_counter = 0;
+  float cached_player[3] = {0.0f, 0.0f, 0.0f};
+  uint32_t tier_n[3] = {0, 0, 0};  // cached near / mid / far list sizes
+  uint32_t tier_version = 0;       // bumped by each reclassification
+  uint64_t work_tier_key = 0;      // tier lists the work set holds (0: none)
+  uint32_t tier_cap = 0;
+  uint32_t* tier_list[3] = {nullptr, nullptr, nullptr};
+  uint8_t* tier_cls = nullptr;
+  float* tier_dist = nullptr;
+  uint32_t* tier_keys[2] = {nullptr, nullptr};
+  uint32_t* tier_vals = nullptr;
+  uint32_t* tier_bcnt = nullptr;  // [tiles][3] counts, then [tiles][3] offsets, then totals
+  void* tier_sort_tmp = nullptr;
+  size_t tier_sort_bytes = 0;
   // scenes
   std::vector<aemh_scene> scenes_h;  // batch table as given
   bool batch_mode = false;
@@ -450,6 +469,18 @@ aemh_status time_launches(aemh_ctx* c, int reps, float* ms, Fn&& fn) {
 
 aemh_status flush_work(aemh_ctx* c);
 
+// Device index list of the work set with room for n entries.
+aemh_status ensure_idx(aemh_ctx* c, uint32_t n) {
+  const size_t need = n ? n : 1;
+  if (c->idx_d && c->idx_cap >= need) return AEMH_OK;
+  if (c->idx_d) hipFree(c->idx_d);
+  c->idx_d = nullptr;
+  c->idx_cap = 0;
+  HIPTRY(c, hipMalloc(&c->idx_d, need * 4));
+  c->idx_cap = need;
+  return AEMH_OK;
+}
+
 // Device copy of a slot list (or permutation) for the lifecycle kernels.
 aemh_status upload_slots(aemh_ctx* c, const uint32_t* slots, size_t n) {
   if (n > c->slots_cap) {
@@ -582,7 +613,7 @@ aemh_status check_overflow(aemh_ctx* c) {
   return AEMH_OK;
 }
 
-aemh_status run_substeps(aemh_ctx* c, uint32_t k) {
+aemh_status run_substeps(aemh_ctx* c, uint32_t k, bool allow_graph = true) {
   if (!c->mats_d) return fail(c, AEMH_ERR_STATE, "materials not set");
   if (c->n_work == 0) return AEMH_OK;
   if (c->shape.p2g_partials) {
@@ -592,7 +623,7 @@ aemh_status run_substeps(aemh_ctx* c, uint32_t k) {
   // Steps replay the captured substep graph.  With profiling on they launch
   // eagerly with an event pair around each kernel section: event-record
   // nodes inside a graph cost the replay more than eager launches do.
-  const bool graph = c->use_graph && !c->prof;
+  const bool graph = allow_graph && c->use_graph && !c->prof;
   if (graph && !c->graph_valid) {
     aemh_status s = build_graphs(c);
     if (s != AEMH_OK) return s;
@@ -712,6 +743,121 @@ aemh_status download_aos(aemh_ctx* c, float* host, float* const* src, int comps,
   return AEMH_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// LOD tier orchestrator helpers (simulation.rs:480-700)
+// ---------------------------------------------------------------------------
+aemh_status ensure_tier(aemh_ctx* c) {
+  const uint32_t cap = c->canon.cap;
+  if (c->tier_cap == cap && c->tier_cls) return AEMH_OK;
+  for (void* p : {(void*)c->tier_list[0], (void*)c->tier_list[1], (void*)c->tier_list[2],
+                  (void*)c->tier_cls, (void*)c->tier_dist, (void*)c->tier_keys[0],
+                  (void*)c->tier_keys[1], (void*)c->tier_vals, (void*)c->tier_bcnt,
+                  c->tier_sort_tmp})
+    if (p) hipFree(p);
+  c->tier_cap = 0;
+  for (int t = 0; t < 3; t++) c->tier_n[t] = 0;
+  const size_t n = cap ? cap : 1;
+  for (int t = 0; t < 3; t++) HIPTRY(c, hipMalloc(&c->tier_list[t], n * 4));
+  HIPTRY(c, hipMalloc(&c->tier_cls, n));
+  HIPTRY(c, hipMalloc(&c->tier_dist, n * 4));
+  HIPTRY(c, hipMalloc(&c->tier_keys[0], n * 4));
+  HIPTRY(c, hipMalloc(&c->tier_keys[1], n * 4));
+  HIPTRY(c, hipMalloc(&c->tier_vals, n * 4));
+  HIPTRY(c, hipMalloc(&c->tier_bcnt, ((size_t)select_tiles((uint32_t)n) * 6 + 4) * 4));
+  c->tier_sort_bytes = sort_pairs_temp_bytes((uint32_t)n);
+  HIPTRY(c, hipMalloc(&c->tier_sort_tmp, c->tier_sort_bytes ? c->tier_sort_bytes : 4));
+  c->tier_cap = cap;
+  return AEMH_OK;
+}
+
+// Ordered compaction through the tile counters; *tot3 (host) gets the sizes.
+aemh_status tier_select(aemh_ctx* c, const uint8_t* cls, const uint32_t* src, const float* key,
+                        uint32_t n, uint32_t* out0, uint32_t* out1, uint32_t* out2,
+                        uint32_t* key0, uint32_t tot3[3]) {
+  const size_t tiles = select_tiles(n ? n : 1);
+  uint32_t* bcnt = c->tier_bcnt;
+  uint32_t* boff = bcnt + tiles * 3;
+  uint32_t* tot = boff + tiles * 3;
+  launch_select(c->stream, cls, src, key, n, bcnt, boff, tot, out0, out1, out2, key0);
+  CHECK_LAUNCH(c);
+  HIPTRY(c, hipMemcpyAsync(tot3, tot, 12, hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  return AEMH_OK;
+}
+
+// LOD reclassification (simulation.rs:495-531): near / mid / far lists of the
+// active slots below count in ascending slot order; a near list longer than
+// max_near_particles keeps its max_near closest (ties: lower slot first, as
+// a stable sort by distance gives) and the rest join the mid list.
+aemh_status tier_classify(aemh_ctx* c, const float player[3]) {
+  const aemh_lod_config& L = c->lod;
+  const float near2 = L.near_field_distance * L.near_field_distance;
+  const float mid2 = L.mid_field_distance * L.mid_field_distance;
+  const uint32_t count = c->mpss_count;
+  aemh_status s = flush_work(c);  // positions as the last step left them
+  if (s != AEMH_OK) return s;
+  launch_lod_classify(c->stream, c->canon.pb, c->life.active, count, player, near2, mid2,
+                      c->tier_cls, c->tier_dist);
+  CHECK_LAUNCH(c);
+  uint32_t n3[3] = {0, 0, 0};
+  s = tier_select(c, c->tier_cls, nullptr, c->tier_dist, count, c->tier_list[0],
+                              c->tier_list[1], c->tier_list[2], c->tier_keys[0], n3);
+  if (s != AEMH_OK) return s;
+  const uint32_t max_near = L.max_near_particles;
+  if (n3[0] > max_near) {
+    HIPTRY(c, launch_sort_pairs(c->stream, c->tier_sort_tmp, c->tier_sort_bytes, c->tier_keys[0],
+                                c->tier_keys[1], c->tier_list[0], c->tier_vals, n3[0]));
+    if (max_near)
+      HIPTRY(c, hipMemcpyAsync(c->tier_list[0], c->tier_vals, (size_t)max_near * 4,
+                               hipMemcpyDeviceToDevice, c->stream));
+    HIPTRY(c, hipMemcpyAsync(c->tier_list[1] + n3[1], c->tier_vals + max_near,
+                             (size_t)(n3[0] - max_near) * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    n3[1] += n3[0] - max_near;
+    n3[0] = max_near;
+  }
+  for (int t = 0; t < 3; t++) c->tier_n[t] = n3[t];
+  c->tier_version += 1;
+  return AEMH_OK;
+}
+
+// Work set = the given device lists back to back (gathered from the
+// canonical slots); reused while the same lists are asked for again.
+aemh_status tier_set_work(aemh_ctx* c, uint64_t key, const uint32_t* const* lists,
+                          const uint32_t* counts, int nl) {
+  if (c->work_valid && c->work_tier_key == key) return AEMH_OK;
+  aemh_status s = flush_work(c);
+  if (s != AEMH_OK) return s;
+  uint32_t n = 0;
+  for (int k = 0; k < nl; k++) n += counts[k];
+  const int storage = (int)c->cfg.storage;
+  for (int k = 0; k < 2; k++)
+    if ((s = alloc_store(c, c->work[k], n ? n : 1, storage)) != AEMH_OK) return s;
+  if ((s = ensure_idx(c, n)) != AEMH_OK) return s;
+  uint32_t off = 0;
+  for (int k = 0; k < nl; k++) {
+    if (counts[k])
+      HIPTRY(c, hipMemcpyAsync(c->idx_d + off, lists[k], (size_t)counts[k] * 4,
+                               hipMemcpyDeviceToDevice, c->stream));
+    off += counts[k];
+  }
+  launch_gather_work(c->stream, c->canon.pb, c->work[0].pb, c->idx_d, n, storage);
+  CHECK_LAUNCH(c);
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  c->n_work = n;
+  c->cur = 0;
+  c->idx_h.clear();
+  c->idx_explicit = false;
+  c->work_default = false;
+  c->work_valid = true;
+  c->work_dirty = false;
+  c->batch_mode = false;
+  c->work_tier_key = key;
+  invalidate_graphs(c);
+  return AEMH_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -776,7 +922,11 @@ aemh_status aemh_destroy(aemh_ctx* c) {
                   (void*)c->sort.bslot, (void*)c->grid.partf, (void*)c->grid.partq,
                   (void*)c->life.lifetime, (void*)c->life.age, (void*)c->life.active,
                   (void*)c->life_tmp.lifetime, (void*)c->life_tmp.age, (void*)c->life_tmp.active,
-                  c->canon_tmp.slab, (void*)c->slots_d, (void*)c->expired_d})
+                  c->canon_tmp.slab, (void*)c->slots_d, (void*)c->expired_d,
+                  (void*)c->tier_list[0], (void*)c->tier_list[1], (void*)c->tier_list[2],
+                  (void*)c->tier_cls, (void*)c->tier_dist, (void*)c->tier_keys[0],
+                  (void*)c->tier_keys[1], (void*)c->tier_vals, (void*)c->tier_bcnt,
+                  c->tier_sort_tmp})
     if (p) hipFree(p);
   if (c->flag_h) hipHostFree(c->flag_h);
   hipStreamDestroy(c->stream);
@@ -876,6 +1026,8 @@ aemh_status aemh_upload_particles(aemh_ctx* c, uint32_t n, const float* pos3, co
   c->work_dirty = false;
   c->idx_explicit = false;
   c->mpss = false;
+  c->work_tier_key = 0;
+  for (int t = 0; t < 3; t++) c->tier_n[t] = 0;
   if (c->sort.counters) HIPTRY(c, hipMemset(c->sort.counters, 0, 64));
   return AEMH_OK;
 }
@@ -937,10 +1089,9 @@ aemh_status aemh_set_indices(aemh_ctx* c, const uint32_t* idx, uint32_t n_idx) {
   const int storage = (int)c->cfg.storage;
   for (int k = 0; k < 2; k++)
     if ((s = alloc_store(c, c->work[k], n_idx ? n_idx : 1, storage)) != AEMH_OK) return s;
-  if (c->idx_d) hipFree(c->idx_d);
-  c->idx_d = nullptr;
-  HIPTRY(c, hipMalloc(&c->idx_d, (size_t)(n_idx ? n_idx : 1) * 4));
+  if ((s = ensure_idx(c, n_idx)) != AEMH_OK) return s;
   if (n_idx) HIPTRY(c, hipMemcpy(c->idx_d, idx, (size_t)n_idx * 4, hipMemcpyHostToDevice));
+  c->work_tier_key = 0;
   c->idx_h.assign(idx, idx + n_idx);
   c->idx_explicit = true;
   c->work_default = false;
@@ -1287,6 +1438,8 @@ aemh_status aemh_mpss_reserve(aemh_ctx* c, uint32_t capacity) {
   c->free_list.clear();
   c->mpss_count = 0;
   c->mpss = true;
+  c->work_tier_key = 0;
+  for (int t = 0; t < 3; t++) c->tier_n[t] = 0;
   c->work_valid = false;
   c->work_dirty = false;
   c->idx_explicit = false;
@@ -1482,6 +1635,204 @@ aemh_status aemh_mpss_state(aemh_ctx* c, uint32_t* count, uint32_t* capacity, ui
   if (count) *count = c->mpss_count;
   if (capacity) *capacity = c->canon.cap;
   if (active) memcpy(active, c->act_h.data(), c->canon.cap);
+  return AEMH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// LOD tier orchestrator (simulation.rs:480-700)
+// ---------------------------------------------------------------------------
+void aemh_lod_config_default(aemh_lod_config* L) {
+  if (!L) return;
+  memset(L, 0, sizeof(*L));
+  // PhysicsLod::default (simulation.rs:35-48)
+  L->near_field_distance = 50.0f;
+  L->mid_field_distance = 200.0f;
+  L->far_field_update_interval = 60;
+  L->max_near_particles = 10000;
+  L->mid_interval = 12;  // MID_INTERVAL (:586)
+  // near grid: next_power_of_two(ceil(2 near / 1.6)) nodes of 1.6 (:219-227)
+  L->near_dx = 1.6f;
+  uint32_t n = (uint32_t)ceilf(2.0f * L->near_field_distance / L->near_dx);
+  uint32_t p = 1;
+  while (p < n) p <<= 1;
+  L->near_n = p;
+  // mid grid: 16 nodes spanning 2 mid (:234-241)
+  L->mid_n = 16;
+  L->mid_dx = (2.0f * L->mid_field_distance) / (float)L->mid_n;
+}
+
+aemh_status aemh_tier_set_config(aemh_ctx* c, const aemh_lod_config* L) {
+  if (!c || !L) return fail(c, AEMH_ERR_ARG, "null argument");
+  if (!(L->near_field_distance >= 0.0f) || !(L->mid_field_distance >= 0.0f) ||
+      L->mid_interval == 0 || L->near_n == 0 || L->mid_n == 0 || !(L->near_dx > 0.0f) ||
+      !(L->mid_dx > 0.0f))
+    return fail(c, AEMH_ERR_ARG, "bad LOD configuration");
+  c->lod = *L;
+  c->lod_set = true;
+  // A new manager: counters, cached player position and lists start over
+  // (SimulationManager::new, simulation.rs:311-317).
+  c->field_counter = 0;
+  c->lod_counter = 0;
+  for (int a = 0; a < 3; a++) c->cached_player[a] = 0.0f;
+  for (int t = 0; t < 3; t++) c->tier_n[t] = 0;
+  c->work_tier_key = 0;
+  return AEMH_OK;
+}
+
+aemh_status aemh_tier_tick(aemh_ctx* c, const float player[3], float dt, aemh_tier_stats* out) {
+  aemh_status s = mpss_ready(c);
+  if (s != AEMH_OK) return s;
+  if (!player) return fail(c, AEMH_ERR_ARG, "null player position");
+  if (!c->lod_set) {
+    aemh_lod_config_default(&c->lod);
+    c->lod_set = true;
+  }
+  if (c->cfg.substeps == 0) return fail(c, AEMH_ERR_ARG, "substeps must be >= 1");
+  aemh_tier_stats st{};
+  c->field_counter += 1u;  // wrapping_add (:474)
+  if (c->mpss_count == 0) {
+    // Empty buffer (:692-697): gravity, integration and ageing visit active
+    // slots only, and there are none.
+    if (out) *out = st;
+    return AEMH_OK;
+  }
+  if ((s = ensure_tier(c)) != AEMH_OK) return s;
+  const aemh_lod_config& L = c->lod;
+  // Reclassify when the player moved more than 5 m, the near cache is
+  // empty, or every 8th tick (:488-494).
+  const float mx = player[0] - c->cached_player[0];
+  const float my = player[1] - c->cached_player[1];
+  const float mz = player[2] - c->cached_player[2];
+  const bool moved = mx * mx + my * my + mz * mz > 25.0f;
+  c->lod_counter += 1u;
+  if (moved || c->tier_n[0] == 0 || c->lod_counter % 8u == 0) {
+    for (int a = 0; a < 3; a++) c->cached_player[a] = player[a];
+    if ((s = tier_classify(c, player)) != AEMH_OK) return s;
+    st.reclassified = 1;
+  }
+  // Moving-window origins: player - nx dx / 2 (:540-556).
+  const float near_half = (float)L.near_n * L.near_dx * 0.5f;
+  const float mid_half = (float)L.mid_n * L.mid_dx * 0.5f;
+  for (int a = 0; a < 3; a++) {
+    st.near_origin[a] = player[a] - near_half;
+    st.mid_origin[a] = player[a] - mid_half;
+  }
+  const bool run_near = c->tier_n[0] > 0;
+  const bool run_mid = c->field_counter % L.mid_interval == 0 && c->tier_n[1] > 0;
+  const uint32_t far_interval = L.far_field_update_interval ? L.far_field_update_interval : 1u;
+  const bool run_far = c->field_counter % far_interval == 0;
+  if (run_near || run_mid) {
+    // The near step and the mid tier's velocity-only step share no
+    // particles and no grid, so both scenes advance in one batched launch
+    // per kernel phase.
+    const uint32_t* lists[2];
+    uint32_t counts[2];
+    int nl = 0;
+    std::vector<aemh_scene> sc;
+    const float sub = (float)c->cfg.substeps;
+    if (run_near) {
+      aemh_scene a{};
+      a.first = 0;
+      a.count = c->tier_n[0];
+      a.nx = a.ny = a.nz = L.near_n;
+      a.dx = L.near_dx;
+      for (int k = 0; k < 3; k++) a.origin[k] = st.near_origin[k];
+      a.dt = dt / sub;  // mpm_solver.rs:1075
+      a.velocity_only = 0;
+      a.clamp_mode = c->cfg.clamp_mode;
+      sc.push_back(a);
+      lists[nl] = c->tier_list[0];
+      counts[nl++] = c->tier_n[0];
+    }
+    if (run_mid) {
+      aemh_scene b{};
+      b.first = run_near ? c->tier_n[0] : 0;
+      b.count = c->tier_n[1];
+      b.nx = b.ny = b.nz = L.mid_n;
+      b.dx = L.mid_dx;
+      for (int k = 0; k < 3; k++) b.origin[k] = st.mid_origin[k];
+      const float mid_dt = dt * (float)L.mid_interval;  // :596
+      b.dt = mid_dt / sub;                               // mpm_solver.rs:1347
+      b.velocity_only = 1;
+      b.clamp_mode = c->cfg.clamp_mode;
+      sc.push_back(b);
+      lists[nl] = c->tier_list[1];
+      counts[nl++] = c->tier_n[1];
+    }
+    const uint64_t key = ((uint64_t)c->tier_version << 2) | (run_near ? 1u : 0u) |
+                         (run_mid ? 2u : 0u);
+    if ((s = tier_set_work(c, key, lists, counts, nl)) != AEMH_OK) return s;
+    if ((s = set_scene_table(c, sc)) != AEMH_OK) return s;
+    // Eager launches: the scene set changes from tick to tick.
+    if ((s = run_substeps(c, c->cfg.substeps, false)) != AEMH_OK) return s;
+  }
+  if (run_far && c->tier_n[2]) {
+    if ((s = flush_work(c)) != AEMH_OK) return s;
+    launch_far_update(c->stream, c->canon.pb, c->tier_list[2], c->tier_n[2],
+                      dt * (float)far_interval);
+    CHECK_LAUNCH(c);
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+  }
+  uint32_t ne = 0;
+  // mpm_step_parallel_with_grid ages the buffer (mpm_solver.rs:1083), then
+  // the tick does once more (:692).
+  if (run_near) {
+    if ((s = aemh_mpss_update_lifetimes(c, dt, nullptr, 0, &ne)) != AEMH_OK) return s;
+    st.expired += ne;
+  }
+  if ((s = aemh_mpss_update_lifetimes(c, dt, nullptr, 0, &ne)) != AEMH_OK) return s;
+  st.expired += ne;
+  st.near = c->tier_n[0];
+  st.mid = c->tier_n[1];
+  st.far = c->tier_n[2];
+  st.near_ran = run_near;
+  st.mid_ran = run_mid;
+  st.far_ran = run_far;
+  if (out) *out = st;
+  return AEMH_OK;
+}
+
+aemh_status aemh_tier_lists(aemh_ctx* c, uint32_t* near, uint32_t* mid, uint32_t* far,
+                            uint32_t* counts3) {
+  aemh_status s = mpss_ready(c);
+  if (s != AEMH_OK) return s;
+  uint32_t* dst[3] = {near, mid, far};
+  for (int t = 0; t < 3; t++) {
+    if (counts3) counts3[t] = c->tier_n[t];
+    if (dst[t] && c->tier_n[t])
+      HIPTRY(c, hipMemcpy(dst[t], c->tier_list[t], (size_t)c->tier_n[t] * 4,
+                          hipMemcpyDeviceToHost));
+  }
+  return AEMH_OK;
+}
+
+aemh_status aemh_tier_render(aemh_ctx* c, uint32_t tier, uint32_t take, float* pos3,
+                             uint32_t* slots, uint32_t* n_out) {
+  aemh_status s = mpss_ready(c);
+  if (s != AEMH_OK) return s;
+  if (tier > 2 || !n_out) return fail(c, AEMH_ERR_ARG, "tier must be 0..2 and n_out given");
+  *n_out = 0;
+  const uint32_t n = std::min(take, c->tier_n[tier]);
+  if (n == 0) return AEMH_OK;
+  if ((s = flush_work(c)) != AEMH_OK) return s;
+  launch_live_filter(c->stream, c->tier_list[tier], n, c->life.active, c->mpss_count,
+                     c->tier_cls);
+  CHECK_LAUNCH(c);
+  uint32_t n3[3] = {0, 0, 0};
+  if ((s = tier_select(c, c->tier_cls, c->tier_list[tier], nullptr, n, c->tier_vals, nullptr,
+                       nullptr, nullptr, n3)) != AEMH_OK)
+    return s;
+  const uint32_t m = n3[0];
+  if (m && pos3) {
+    if ((s = ensure_staging(c, (size_t)m * 12)) != AEMH_OK) return s;
+    launch_gather_pos(c->stream, c->canon.pb, c->tier_vals, m, c->staging);
+    CHECK_LAUNCH(c);
+    HIPTRY(c, hipMemcpyAsync(pos3, c->staging, (size_t)m * 12, hipMemcpyDeviceToHost, c->stream));
+  }
+  if (m && slots)
+    HIPTRY(c, hipMemcpyAsync(slots, c->tier_vals, (size_t)m * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  *n_out = m;
   return AEMH_OK;
 }
 

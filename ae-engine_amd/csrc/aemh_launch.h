@@ -56,6 +56,25 @@ void launch_u16_to_slots(hipStream_t st, const uint16_t* v, uint16_t* dst, const
 void launch_slots_to_u16(hipStream_t st, const uint16_t* src, uint16_t* v, const uint32_t* slots,
                          uint32_t n);
 
+// ---- k_tier.hip: LOD tier orchestrator ------------------------------------
+// Tiles of the ordered compaction (bcnt / boff hold 3 words per tile).
+uint32_t select_tiles(uint32_t n);
+void launch_lod_classify(hipStream_t st, PBuf P, const uint8_t* active, uint32_t count,
+                         const float player[3], float near2, float mid2, uint8_t* cls,
+                         float* dist);
+void launch_live_filter(hipStream_t st, const uint32_t* list, uint32_t n, const uint8_t* active,
+                        uint32_t count, uint8_t* cls);
+// Items of class 0/1/2 (cls, one byte per item) to out0/out1/out2 in item
+// order; tot[3] gets the class sizes.  Null outputs are skipped.
+void launch_select(hipStream_t st, const uint8_t* cls, const uint32_t* src, const float* key,
+                   uint32_t n, uint32_t* bcnt, uint32_t* boff, uint32_t* tot, uint32_t* out0,
+                   uint32_t* out1, uint32_t* out2, uint32_t* key0);
+size_t sort_pairs_temp_bytes(uint32_t n);
+hipError_t launch_sort_pairs(hipStream_t st, void* tmp, size_t tmp_bytes, const uint32_t* kin,
+                             uint32_t* kout, const uint32_t* vin, uint32_t* vout, uint32_t n);
+void launch_far_update(hipStream_t st, PBuf P, const uint32_t* far, uint32_t n, float far_dt);
+void launch_gather_pos(hipStream_t st, PBuf P, const uint32_t* list, uint32_t n, float* out3);
+
 // ---- k_fast.hip (FP32 mode) and k_det.hip (deterministic mode) -----------
 #define AEMH_DECLARE_MODE(NS)                                                               \
   namespace NS {                                                                            \

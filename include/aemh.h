@@ -177,6 +177,63 @@ aemh_status aemh_mpss_read(aemh_ctx* ctx, const uint32_t* slots, uint32_t n,
 aemh_status aemh_mpss_state(aemh_ctx* ctx, uint32_t* count, uint32_t* capacity,
                             uint8_t* active);
 
+/* LOD tier orchestrator (ae_engine/src/managers/simulation.rs:480-700) -- */
+/* PhysicsLod (simulation.rs:27-48) plus the tier grids and the mid cadence. */
+typedef struct {
+  float near_field_distance;          /* 50 m                                  */
+  float mid_field_distance;           /* 200 m                                 */
+  uint32_t far_field_update_interval; /* 60 ticks (0 acts as 1, :594)        */
+  uint32_t max_near_particles;        /* 10,000                               */
+  uint32_t mid_interval;              /* MID_INTERVAL = 12 ticks (:586)       */
+  uint32_t near_n;                    /* next_pow2(ceil(2 near / 1.6)) = 64  */
+  float near_dx;                      /* 1.6 m (:219-227)                     */
+  uint32_t mid_n;                     /* 16 (:234)                            */
+  float mid_dx;                       /* 2 mid / 16 = 25 m (:235)             */
+  uint32_t reserved[3];
+} aemh_lod_config;
+
+/* What one tick did: LodStats (cached list sizes, :533-536), which tiers
+ * ran, the moving-window origins (:540-556), and the slots the tick's
+ * update_lifetimes calls killed. */
+typedef struct {
+  uint32_t near, mid, far;
+  uint32_t reclassified;
+  uint32_t near_ran, mid_ran, far_ran;
+  uint32_t expired;
+  float near_origin[3];
+  float mid_origin[3];
+} aemh_tier_stats;
+
+void aemh_lod_config_default(aemh_lod_config* lod);
+/* Sets the tier configuration and starts the tick counters, the cached
+ * player position (origin) and the cached lists over, as a new
+ * SimulationManager does (simulation.rs:311-317).  Without this call the
+ * first tick uses aemh_lod_config_default. */
+aemh_status aemh_tier_set_config(aemh_ctx* ctx, const aemh_lod_config* lod);
+/* The MPSS part of SimulationManager::update_fields_and_particles
+ * (simulation.rs:474-697) on the resident buffer: reclassification when the
+ * player moved > 5 m, the near list is empty or every 8th tick (near / mid /
+ * far by distance, near culled to the max_near closest with the rest moved
+ * to mid); moving-window origins; the near full step
+ * (mpm_step_parallel_with_grid, which ages lifetimes) and, every
+ * mid_interval ticks, the mid velocity-only step with dt x mid_interval,
+ * both scenes in one batched launch; every far_field_update_interval ticks
+ * the far gravity update (velocity and height; the coarse temperature field
+ * is not part of this library); then update_lifetimes(dt).  Material and
+ * config come from aemh_set_materials / aemh_set_config.  stats may be NULL. */
+aemh_status aemh_tier_tick(aemh_ctx* ctx, const float player[3], float dt,
+                           aemh_tier_stats* stats);
+/* Cached lists (NULL: skipped; each array needs room for its count) and
+ * their sizes counts3[3] (NULL: skipped). */
+aemh_status aemh_tier_lists(aemh_ctx* ctx, uint32_t* near, uint32_t* mid, uint32_t* far,
+                            uint32_t* counts3);
+/* Render readback (ae_engine/src/lib.rs:1891-1936): of the first `take`
+ * entries of cached list `tier` (0 near, 1 mid, 2 far), those still live
+ * (slot < count and active), in list order: positions pos3[n][3] and slots
+ * (either may be NULL); *n_out gets n. */
+aemh_status aemh_tier_render(aemh_ctx* ctx, uint32_t tier, uint32_t take, float* pos3,
+                             uint32_t* slots, uint32_t* n_out);
+
 /* Steps ----------------------------------------------------------------- */
 /* mpm_step_parallel_with_grid (mpm_solver.rs:1063-1084): substeps of
  * total_dt/substeps over the active set on the context grid. */
