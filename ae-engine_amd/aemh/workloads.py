@@ -14,8 +14,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (ACCUM_F32, ACCUM_Q32_32, CLAMP_AUTO, NEO_HOOKEAN, NEWTONIAN_FLUID, SNOW,
-               STORE_F32, Material, Scene, default_config, uniform_box)
+from . import (ACCUM_F32, ACCUM_Q32_32, CLAMP_AUTO, DRUCKER_PRAGER, NEO_HOOKEAN,
+               NEWTONIAN_FLUID, SNOW, STORE_F32, Material, Scene, default_config, uniform_box)
 
 SEED_BASE = 42
 
@@ -176,3 +176,57 @@ def load_engine(eng, wl: Workload):
     eng.upload(wl.pos, mass=wl.mass, mat=wl.mat)
     if wl.scenes:
         eng.set_scenes(wl.scenes)
+
+
+def tier_materials():
+    """The engine's material kinds (simulation.rs:243-294: elastic, fluid,
+    sand) with E scaled so each tier is CFL-stable at dt 1/60 s, dx 1.6 m
+    (wave speed <= 26 m/s); the steel entry is left out."""
+    return [
+        Material(young_modulus=1e5, poisson_ratio=0.3, yield_stress=1e4, hardening=0.01,
+                 density=1000.0, friction_angle=35.0, cohesion=0.0, model=NEO_HOOKEAN),
+        Material(young_modulus=2e5, poisson_ratio=0.45, yield_stress=0.0, hardening=0.0,
+                 density=1000.0, friction_angle=0.0, cohesion=0.0, model=NEWTONIAN_FLUID),
+        Material(young_modulus=1e6, poisson_ratio=0.3, yield_stress=0.0, hardening=0.0,
+                 density=1600.0, friction_angle=35.0, cohesion=0.0, model=DRUCKER_PRAGER),
+    ]
+
+
+@dataclass
+class TierWorld:
+    name: str
+    capacity: int
+    pos: np.ndarray
+    mat: np.ndarray
+    materials: list
+
+    @property
+    def n(self):
+        return self.pos.shape[0]
+
+
+def _shell(rng, n, r0, r1):
+    """n points uniform in volume in the upper half shell r0 <= r < r1."""
+    u = rng.random(n)
+    r = np.cbrt(u * (r1 ** 3 - r0 ** 3) + r0 ** 3)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d[:, 1] = np.abs(d[:, 1])
+    return (d * r[:, None]).astype(np.float32)
+
+
+def engine_tiers(n_live=2_000_000, capacity=3_000_000, n_near=20_000, n_mid=490_000, seed=6):
+    """The engine's resident buffer at scale: MpssBuffer::new(3_000_000)
+    (simulation.rs:301) holding n_live particles around a player at the
+    origin -- n_near within the 50 m near radius (twice max_near_particles,
+    so the cull runs), n_mid in the 50-200 m mid shell (the 490k-particle
+    velocity-only tier of ARCHITECTURE_V7_CRITIQUE.md:549), the rest far
+    (200-400 m); materials uniform over tier_materials()."""
+    rng = np.random.default_rng(SEED_BASE + seed)
+    n_far = n_live - n_near - n_mid
+    pos = np.concatenate([_shell(rng, n_near, 0.0, 50.0), _shell(rng, n_mid, 50.0, 200.0),
+                          _shell(rng, n_far, 200.0, 400.0)])
+    perm = rng.permutation(n_live)
+    mats = tier_materials()
+    return TierWorld(name=f"engine_tiers_{n_live}", capacity=capacity, pos=pos[perm],
+                     mat=rng.integers(0, len(mats), n_live).astype(np.uint16), materials=mats)

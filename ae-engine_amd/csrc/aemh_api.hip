@@ -557,7 +557,7 @@ void enqueue_substep(aemh_ctx* c, int cur) {
   const int q32 = c->cfg.accum == AEMH_ACCUM_Q32_32;
   hipStream_t st = c->stream;
   timed(c, "step_begin", [&] { launch_step_begin(st, c->sort); });
-  timed(c, "count", [&] { launch_count(st, a, c->scd_d, ns, n, c->sort); });
+  timed(c, "count", [&] { launch_count(st, a, c->scd_d, ns, n, c->sort, c->nb_total); });
   timed(c, "scan", [&] { launch_scan(st, c->scd_d, ns, c->sort, c->nb_total); });
   timed(c, "scatter", [&] { launch_scatter(st, n, c->sort); });
   timed(c, "clear", [&] { launch_clear(st, c->sort, c->grid, q32, c->shape.upd_wgs); });
@@ -779,7 +779,9 @@ aemh_status tier_select(aemh_ctx* c, const uint8_t* cls, const uint32_t* src, co
   uint32_t* bcnt = c->tier_bcnt;
   uint32_t* boff = bcnt + tiles * 3;
   uint32_t* tot = boff + tiles * 3;
-  launch_select(c->stream, cls, src, key, n, bcnt, boff, tot, out0, out1, out2, key0);
+  timed(c, "lod_select", [&] {
+    launch_select(c->stream, cls, src, key, n, bcnt, boff, tot, out0, out1, out2, key0);
+  });
   CHECK_LAUNCH(c);
   HIPTRY(c, hipMemcpyAsync(tot3, tot, 12, hipMemcpyDeviceToHost, c->stream));
   HIPTRY(c, hipStreamSynchronize(c->stream));
@@ -797,8 +799,10 @@ aemh_status tier_classify(aemh_ctx* c, const float player[3]) {
   const uint32_t count = c->mpss_count;
   aemh_status s = flush_work(c);  // positions as the last step left them
   if (s != AEMH_OK) return s;
-  launch_lod_classify(c->stream, c->canon.pb, c->life.active, count, player, near2, mid2,
-                      c->tier_cls, c->tier_dist);
+  timed(c, "lod_classify", [&] {
+    launch_lod_classify(c->stream, c->canon.pb, c->life.active, count, player, near2, mid2,
+                        c->tier_cls, c->tier_dist);
+  });
   CHECK_LAUNCH(c);
   uint32_t n3[3] = {0, 0, 0};
   s = tier_select(c, c->tier_cls, nullptr, c->tier_dist, count, c->tier_list[0],
@@ -806,8 +810,12 @@ aemh_status tier_classify(aemh_ctx* c, const float player[3]) {
   if (s != AEMH_OK) return s;
   const uint32_t max_near = L.max_near_particles;
   if (n3[0] > max_near) {
-    HIPTRY(c, launch_sort_pairs(c->stream, c->tier_sort_tmp, c->tier_sort_bytes, c->tier_keys[0],
-                                c->tier_keys[1], c->tier_list[0], c->tier_vals, n3[0]));
+    hipError_t e = hipSuccess;
+    timed(c, "near_cull", [&] {
+      e = launch_sort_pairs(c->stream, c->tier_sort_tmp, c->tier_sort_bytes, c->tier_keys[0],
+                            c->tier_keys[1], c->tier_list[0], c->tier_vals, n3[0]);
+    });
+    HIPTRY(c, e);
     if (max_near)
       HIPTRY(c, hipMemcpyAsync(c->tier_list[0], c->tier_vals, (size_t)max_near * 4,
                                hipMemcpyDeviceToDevice, c->stream));
@@ -1192,7 +1200,7 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   hipStream_t st = c->stream;
   LaunchShape& ls = c->shape;
   launch_step_begin(st, c->sort);
-  launch_count(st, a, c->scd_d, ns, n, c->sort);
+  launch_count(st, a, c->scd_d, ns, n, c->sort, c->nb_total);
   launch_scan(st, c->scd_d, ns, c->sort, c->nb_total);
   launch_scatter(st, n, c->sort);
   CHECK_LAUNCH(c);
@@ -1527,7 +1535,9 @@ aemh_status aemh_mpss_update_lifetimes(aemh_ctx* c, float dt, uint32_t* expired_
   // update_lifetimes (mpss.rs:431-445): active slots below count age by dt;
   // those at or past their lifetime are killed in ascending slot order.
   HIPTRY(c, hipMemsetAsync(c->expired_d, 0, 4, c->stream));
-  launch_mpss_age(c->stream, c->life, c->mpss_count, dt, c->expired_d + 1, c->expired_d);
+  timed(c, "lifetimes", [&] {
+    launch_mpss_age(c->stream, c->life, c->mpss_count, dt, c->expired_d + 1, c->expired_d);
+  });
   CHECK_LAUNCH(c);
   uint32_t ne = 0;
   HIPTRY(c, hipMemcpyAsync(&ne, c->expired_d, 4, hipMemcpyDeviceToHost, c->stream));
@@ -1768,8 +1778,10 @@ aemh_status aemh_tier_tick(aemh_ctx* c, const float player[3], float dt, aemh_ti
   }
   if (run_far && c->tier_n[2]) {
     if ((s = flush_work(c)) != AEMH_OK) return s;
-    launch_far_update(c->stream, c->canon.pb, c->tier_list[2], c->tier_n[2],
-                      dt * (float)far_interval);
+    timed(c, "far_update", [&] {
+      launch_far_update(c->stream, c->canon.pb, c->tier_list[2], c->tier_n[2],
+                        dt * (float)far_interval);
+    });
     CHECK_LAUNCH(c);
     HIPTRY(c, hipStreamSynchronize(c->stream));
   }
@@ -1782,6 +1794,7 @@ aemh_status aemh_tier_tick(aemh_ctx* c, const float player[3], float dt, aemh_ti
   }
   if ((s = aemh_mpss_update_lifetimes(c, dt, nullptr, 0, &ne)) != AEMH_OK) return s;
   st.expired += ne;
+  if (c->prof) collect_profile(c);
   st.near = c->tier_n[0];
   st.mid = c->tier_n[1];
   st.far = c->tier_n[2];

@@ -21,7 +21,8 @@ struct LaunchShape {
 
 // ---- k_common.hip --------------------------------------------------------
 void launch_step_begin(hipStream_t st, SortDev s);
-void launch_count(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, uint32_t n, SortDev s);
+void launch_count(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, uint32_t n, SortDev s,
+                  uint32_t nb_total);
 // Scan of the block counts; also builds this substep's active-block list.
 void launch_scan(hipStream_t st, const SceneDev* sc, int ns, SortDev s, uint32_t nb);
 void launch_scatter(hipStream_t st, uint32_t n, SortDev s);

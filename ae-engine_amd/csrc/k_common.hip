@@ -46,6 +46,64 @@ __global__ __launch_bounds__(256) void k_count(PBuf cur, const SceneDev* __restr
   if (valid) s.rank[i] = base + (uint32_t)(lane - h);
 }
 
+// Grids with few node blocks (a coarse tier grid holding half a million
+// particles, say): unsorted input would queue hundreds of lanes on each block
+// counter in HBM.  Each workgroup takes 4 rounds of 256 particles, counts
+// them in LDS bins (a run of equal keys inside a wave still takes one LDS
+// atomic), then adds each touched bin to its block counter with one HBM
+// atomic; a particle's rank is that atomic's result plus its LDS rank.
+constexpr uint32_t kCountLdsBins = 8192;
+constexpr int kCountRounds = 4;
+
+__global__ __launch_bounds__(256) void k_count_lds(PBuf cur, const SceneDev* __restrict__ sc,
+                                                   int ns, uint32_t n, SortDev s, uint32_t nb) {
+  __shared__ uint32_t bins[kCountLdsBins];
+  for (uint32_t j = threadIdx.x; j < nb; j += 256u) bins[j] = 0u;
+  __syncthreads();
+  const int lane = __lane_id();
+  const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  uint32_t keys[kCountRounds], loc[kCountRounds];
+  bool own[kCountRounds];
+#pragma unroll
+  for (int r = 0; r < kCountRounds; r++) {
+    const uint32_t i = (blockIdx.x * kCountRounds + r) * 256u + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t key = 0xffffffffu;
+    if (valid) {
+      const SceneDev S = sc[scene_of(sc, ns, i)];
+      const Stencil st = make_stencil(S, cur.x[0][i], cur.x[1][i], cur.x[2][i]);
+      key = block_key(S, st);
+      s.key[i] = key;
+    }
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool head = valid && (lane == 0 || prev != key);
+    const unsigned long long heads = __ballot(head);
+    const int nvalid = __popcll(__ballot(valid));
+    uint32_t old = 0;
+    if (head) {
+      const unsigned long long above = heads & ~le;
+      const int next = above ? (__ffsll((long long)above) - 1) : nvalid;
+      old = atomicAdd(&bins[key], (uint32_t)(next - lane));
+    }
+    const unsigned long long mine = valid ? (heads & le) : 1ull;
+    const int h = 63 - __clzll((long long)mine);
+    const uint32_t base = __shfl(old, h, 64);
+    keys[r] = key;
+    loc[r] = base + (uint32_t)(lane - h);
+    own[r] = head && old == 0u;  // first adder of this bin in the workgroup
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kCountRounds; r++)
+    if (own[r]) bins[keys[r]] = atomicAdd(&s.cnt[keys[r]], bins[keys[r]]);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kCountRounds; r++) {
+    const uint32_t i = (blockIdx.x * kCountRounds + r) * 256u + threadIdx.x;
+    if (i < n) s.rank[i] = bins[keys[r]] + loc[r];
+  }
+}
+
 // Every non-empty base block activates the 2x2x2 node blocks its 6^3 halo
 // can touch (idempotent plain stores, no atomics).
 __device__ __forceinline__ void activate_halo(const SceneDev* __restrict__ sc, int ns, SortDev& s,
@@ -285,8 +343,13 @@ inline unsigned grid_for(uint32_t n, unsigned wg) { return (unsigned)((n + wg - 
 }  // namespace
 
 void launch_step_begin(hipStream_t st, SortDev s) { k_step_begin<<<1, 64, 0, st>>>(s); }
-void launch_count(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, uint32_t n, SortDev s) {
-  if (n) k_count<<<grid_for(n, 256), 256, 0, st>>>(cur, sc, ns, n, s);
+void launch_count(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, uint32_t n, SortDev s,
+                  uint32_t nb_total) {
+  if (!n) return;
+  if (nb_total <= kCountLdsBins)
+    k_count_lds<<<grid_for(n, 256 * kCountRounds), 256, 0, st>>>(cur, sc, ns, n, s, nb_total);
+  else
+    k_count<<<grid_for(n, 256), 256, 0, st>>>(cur, sc, ns, n, s);
 }
 void launch_scan(hipStream_t st, const SceneDev* sc, int ns, SortDev s, uint32_t nb) {
   const uint32_t tiles = (nb + kScanTile - 1) / kScanTile;

@@ -17,7 +17,8 @@ the context stream -- and give the per-kernel durations behind `roofline`
 they are kept out of the window that gives `value`).  The launch shape is
 autotuned once before the warm-up.  `extra` adds the profiled window's own
 throughput, the 16M-particle sparse-grid workload, the deterministic Q32.32
-mode, the batched-scenes speedup and the mixed-precision workload.
+mode, the batched-scenes speedup, the mixed-precision workload and the
+engine's LOD tier tick on a 3M-capacity resident buffer.
 """
 from __future__ import annotations
 
@@ -216,6 +217,126 @@ def extra_batched(W, aemh, steps=5):
             "batched_vs_plain_speedup": el_p / el_b}
 
 
+TIER_KERNELS = ("lod_classify", "lod_select", "near_cull", "count", "scan", "scatter", "clear",
+                "p2g", "grid_update", "g2p", "far_update", "lifetimes")
+
+
+def extra_tiers(W, aemh, ticks=60, with_cpu=True):
+    """The engine's LOD tier tick (simulation.rs:474-697) on the resident
+    buffer at engine scale; per-tick wall time by kind of tick, the
+    classification pass against HBM, and the oracle's cost of the same tick
+    on the host."""
+    import numpy as np
+    tw = W.engine_tiers()
+    eng = aemh.Engine()
+    eng.set_config(aemh.default_config())  # MpmConfig::default: FP32, 1 substep
+    eng.set_materials(tw.materials)
+    lod = aemh.default_lod_config()
+    eng.tier_set_config(lod)
+    eng.mpss_reserve(tw.capacity)
+    slots = eng.mpss_spawn(tw.n)
+    eng.mpss_write(slots, pos=tw.pos, mat=tw.mat)
+    dt = 1.0 / 60
+    player = np.zeros(3, np.float32)
+    walk = np.array([1.4 * dt, 0.0, 0.0], np.float32)  # walking pace
+    for _ in range(12):  # allocations, first classification, first mid tick
+        player += walk
+        eng.tier_tick(player, dt)
+    kinds = {}
+    t_all = 0.0
+    for _ in range(ticks):
+        player += walk
+        t0 = time.perf_counter()
+        st = eng.tier_tick(player, dt)
+        el = time.perf_counter() - t0
+        t_all += el
+        kind = "mid" if st["mid_ran"] else ("reclassify" if st["reclassified"] else "near")
+        if st["far_ran"]:
+            kind += "+far"
+        kinds.setdefault(kind, []).append(el)
+    eng.profile(True)
+    eng.profile_reset()
+    for _ in range(24):
+        player += walk
+        eng.tier_tick(player, dt)
+    prof = {k: eng.profile_get(k) for k in TIER_KERNELS}
+    eng.profile(False)
+    count = eng.mpss_state()[0]
+    near, mid, far = eng.tier_lists()
+    eng.close()
+    cls_ms = prof["lod_classify"][0]
+    cls_bytes = count * (12 + 1 + 1 + 4)  # pos read, active read, class + distance written
+    out = {"workload": f"{tw.name}: capacity {tw.capacity}, {tw.n} live, tiers near/mid/far "
+                       f"{near.size}/{mid.size}/{far.size} (max_near 10000), player walking 1.4 m/s, "
+                       f"dt 1/60, FP32",
+           "ticks": ticks,
+           "ms_per_tick": round(t_all / ticks * 1e3, 3),
+           "ms_by_tick_kind": {k: {"n": len(v), "avg_ms": round(sum(v) / len(v) * 1e3, 3)}
+                               for k, v in sorted(kinds.items())},
+           "kernel_avg_ms": {k: round(v[0], 4) for k, v in prof.items() if v[1]},
+           "classify_roofline": {"bound": "hbm", "achieved": round(cls_bytes / (cls_ms * 1e-3) / 1e9, 1)
+                                 if cls_ms > 0 else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "algorithmic_bytes": cls_bytes, "avg_ms": round(cls_ms, 4)}}
+    if cls_ms > 0:
+        out["classify_roofline"]["frac"] = round(cls_bytes / (cls_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    if with_cpu:
+        out["cpu_port"] = tier_cpu_port(tw, aemh, lod, near, mid, far, dt)
+    return out
+
+
+def tier_cpu_port(tw, aemh, lod, near, mid, far, dt):
+    """The oracle's cost of one average tick of the same world on the host:
+    classification (vectorized f32, stable sort for the cull) amortized over
+    the 8-tick reclassification, the near step (serial, the reference's path
+    below 50,000 indices) every tick, the velocity-only mid step (serial P2G)
+    every 12 ticks, two lifetime passes per tick.  Timed once each."""
+    import numpy as np
+    from oracle import refcpu as R
+    f32 = np.float32
+    p, m, e = aemh.material_arrays(tw.materials)
+    M = R.Materials(p, m, e)
+    st = R.State(tw.pos, mat=tw.mat)
+    player = np.zeros(3, f32)
+    t0 = time.perf_counter()
+    d = (st.pos - player).astype(f32)
+    d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    nr = np.flatnonzero(d2 < f32(lod.near_field_distance) ** 2)
+    md = np.flatnonzero((d2 >= f32(lod.near_field_distance) ** 2)
+                        & (d2 < f32(lod.mid_field_distance) ** 2))
+    _fr = np.flatnonzero(d2 >= f32(lod.mid_field_distance) ** 2)
+    order = np.argsort(d2[nr], kind="stable")
+    _near_c = nr[order[:lod.max_near_particles]]
+    _mid_c = np.concatenate([md, nr[order[lod.max_near_particles:]]])
+    t_cls = time.perf_counter() - t0
+    half = f32(f32(lod.near_n) * f32(lod.near_dx)) * f32(0.5)
+    cfg_n = R.make_config(lod.near_n, lod.near_n, lod.near_n, float(lod.near_dx),
+                          tuple(float(-half) for _ in range(3)), 1, (0, -9.81, 0), 0.995, 0, 0.0,
+                          R.ACCUM_F32, R.CLAMP_AUTO, threads=1)
+    t0 = time.perf_counter()
+    R.step(cfg_n, M, st, dt, idx=near)
+    t_near = time.perf_counter() - t0
+    mh = f32(f32(lod.mid_n) * f32(lod.mid_dx)) * f32(0.5)
+    cfg_m = R.make_config(lod.mid_n, lod.mid_n, lod.mid_n, float(lod.mid_dx),
+                          tuple(float(-mh) for _ in range(3)), 1, (0, -9.81, 0), 0.995, 0, 0.0,
+                          R.ACCUM_F32, R.CLAMP_AUTO, threads=1)
+    t0 = time.perf_counter()
+    R.step_velocity_only(cfg_m, st, dt * lod.mid_interval, idx=mid)
+    t_mid = time.perf_counter() - t0
+    age = np.zeros(tw.n, f32)
+    life = np.full(tw.n, np.finfo(f32).max, f32)
+    t0 = time.perf_counter()
+    age += f32(dt)
+    _exp = np.flatnonzero(age >= life)
+    t_life = time.perf_counter() - t0
+    per_tick = t_near + t_cls / 8 + t_mid / lod.mid_interval + 2 * t_life
+    return {"ms_per_tick": round(per_tick * 1e3, 3), "cores": 1, "kind": "port",
+            "parts_ms": {"classify": round(t_cls * 1e3, 3), "near_step": round(t_near * 1e3, 3),
+                         "mid_velocity_only_step": round(t_mid * 1e3, 3),
+                         "lifetimes": round(t_life * 1e3, 3)},
+            "sample": "oracle components timed once each on the same world, composed with "
+                      "the tick cadences (classify 1/8, near 1/1, mid 1/12, lifetimes 2/1)"}
+
+
 def probe(W, aemh):
     """Counter-collection run.  The copy moves exactly 256 MiB in and 256 MiB
     out, so FETCH_SIZE / WRITE_SIZE of its dispatch give the KiB each counter
@@ -302,6 +423,8 @@ def main():
         extra["c4"] = extra_batched(W, aemh)
         log("[bench] C5 mixed precision")
         extra["c5"] = extra_c5(W, aemh)
+        log("[bench] LOD tier tick at engine scale")
+        extra["tiers"] = extra_tiers(W, aemh, with_cpu=not args.no_cpu)
     line["extra"] = extra
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
