@@ -638,11 +638,33 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
   const int lane = __lane_id(), wave = t >> 6;
   const uint32_t j = blockIdx.x * blockDim.x + (uint32_t)t;
   const bool valid = j < n;
-  uint32_t p = 0, key = 0xffffffffu;
+  const SceneDev S = sc[scene_of(sc, ns, valid ? j : 0u)];
+  // Every per-particle input is loaded as soon as perm is known: the loads
+  // overlap the tile staging below instead of following its barriers.  The
+  // block key comes from the position (the count kernel's function of the
+  // same x), not from a second dependent load.
+  const uint32_t p = valid ? s.perm[j] : 0u;
+  float px = 0.0f, py = 0.0f, pz = 0.0f, pmass = 0.0f, pjp = 1.0f;
+  uint16_t mi = 0;
+  uint32_t pid = 0;
+  float F[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) F[k] = (k % 4 == 0) ? 1.0f : 0.0f;
   if (valid) {
-    p = s.perm[j];
-    key = s.key[p];
+    px = cur.x[0][p];
+    py = cur.x[1][p];
+    pz = cur.x[2][p];
+    mi = cur.mat[p];
+    pmass = cur.mass[p];
+    pid = cur.id[p];
+    pjp = cur.Jp[p];
+    if (!S.vel_only) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) F[k] = get_f(cur, p, k, STORE);
+    }
   }
+  const Stencil st = make_stencil(S, px, py, pz);
+  const uint32_t key = valid ? block_key(S, st) : 0xffffffffu;
   skey[t] = key;
   __syncthreads();
   const bool head = valid && (t == 0 || skey[t - 1] != key);
@@ -657,7 +679,6 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
   }
   const int slot = (int)(pre + (uint32_t)__popcll(hb & le)) - 1;
   const int ntiles = (int)min(tot, (uint32_t)kG2PTiles);
-  const SceneDev S = sc[scene_of(sc, ns, valid ? j : 0u)];
   if (head && slot < kG2PTiles) {
     const uint32_t lb = key - S.block_off;
     gmeta[slot][0] = (int)(lb % (uint32_t)S.nb[0]) * 4;
@@ -671,36 +692,28 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
     gboff[slot] = S.block_off;
   }
   __syncthreads();
-  for (int e = t; e < ntiles * kTileNodes; e += (int)blockDim.x) {
-    const int ti = e / kTileNodes, li = e - ti * kTileNodes;
-    const int x = gmeta[ti][0] + li % kTileSide, y = gmeta[ti][1] + (li / kTileSide) % kTileSide,
-              z = gmeta[ti][2] + li / (kTileSide * kTileSide);
-    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // outside the grid: never kept
-    if (x < gmeta[ti][3] && y < gmeta[ti][4] && z < gmeta[ti][5]) {
-      uint32_t ux, uy, uz;
-      node_axes(gboff[ti], gmeta[ti][6], gmeta[ti][7], x, y, z, ux, uy, uz);
-      v = G.gv[ux + uy + uz];
+  // Tile node li of every staged tile; outside the grid (v, m) = 0.  A node
+  // with m < 1e-12, which every gather skips, is staged with v = 0, so the
+  // FP32 gather's product with it is zero without a test.
+  for (int li = t; li < kTileNodes; li += (int)blockDim.x) {
+    const int lx = li % kTileSide, ly = (li / kTileSide) % kTileSide,
+              lz = li / (kTileSide * kTileSide);
+    for (int ti = 0; ti < ntiles; ti++) {
+      const int x = gmeta[ti][0] + lx, y = gmeta[ti][1] + ly, z = gmeta[ti][2] + lz;
+      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (x < gmeta[ti][3] && y < gmeta[ti][4] && z < gmeta[ti][5]) {
+        uint32_t ux, uy, uz;
+        node_axes(gboff[ti], gmeta[ti][6], gmeta[ti][7], x, y, z, ux, uy, uz);
+        v = G.gv[ux + uy + uz];
+        if (v.w < 1e-12f) v.x = v.y = v.z = 0.0f;
+      }
+      gtile[ti * kTileNodes + li] = v;
     }
-    gtile[e] = v;
   }
   __syncthreads();
   if (!valid) return;
   const bool use_lds = slot < kG2PTiles;
   const float4* __restrict__ gvp = G.gv;
-  const float px = cur.x[0][p], py = cur.x[1][p], pz = cur.x[2][p];
-  // Every other per-particle input is loaded here too, ahead of the first
-  // store to nxt (which the compiler must assume may alias cur), so these
-  // loads overlap the gather instead of following it.
-  const uint16_t mi = cur.mat[p];
-  const float pmass = cur.mass[p];
-  const uint32_t pid = cur.id[p];
-  const float pjp = cur.Jp[p];
-  float F[9];
-  if (!S.vel_only) {
-#pragma unroll
-    for (int k = 0; k < 9; k++) F[k] = get_f(cur, p, k, STORE);
-  }
-  const Stencil st = make_stencil(S, px, py, pz);
   const int n0 = S.n[0], n1 = S.n[1], n2 = S.n[2];
   const int nb0 = S.nb[0], nb1 = S.nb[1];
   const uint32_t boff = S.block_off;
@@ -880,27 +893,7 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
       sx[a][r] = sy[a][r] = 0.0f;
       wxy[a][r] = wx[r] * wy[a];  // [dj][di]
     }
-  auto gather = [&](auto node_at) {
-#pragma unroll 1
-  for (int dk = 0; dk < 3; dk++) {
-    const float wzk = wz[dk];
-    float t[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int dj = 0; dj < 3; dj++)
-#pragma unroll
-      for (int di = 0; di < 3; di++) {
-        const float w0 = wxy[dj][di] * wzk;
-        const float4 g = node_at(di, dj, dk);
-        // bitwise &: every lane loads the whole node (no load behind a branch)
-        const bool keep = !(w0 < 1e-10f) & ix[di] & iy[dj] & iz[dk] & !(g.w < 1e-12f);
-        const float u[3] = {keep ? w0 * g.x : 0.0f, keep ? w0 * g.y : 0.0f, keep ? w0 * g.z : 0.0f};
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-          sx[di][r] += u[r];
-          sy[dj][r] += u[r];
-          t[r] += u[r];
-        }
-      }
+  auto layer_sums = [&](int dk, const float t[3]) {
     const float dzk = dza[dk], dwzk = dwz[dk];
 #pragma unroll
     for (int r = 0; r < 3; r++) {
@@ -908,12 +901,76 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
       zc[r] += t[r] * dzk;
       zg[r] += t[r] * dwzk;
     }
-  }
   };
-  if (use_lds)
-    gather(lds_node);
-  else
-    gather(hbm_node);
+  if (use_lds) {
+    // Staged tile: node (di, dj, dk) sits at a constant offset from the
+    // particle's base node.  Nodes past the grid's far faces get weight 0
+    // (an axis weight, so w0 = 0 < 1e-10 skips them as the in-grid test
+    // does); nodes with m < 1e-12 were staged with v = 0.  Only the weight
+    // test is left per node: u = w g with w = 0 for a skipped node adds +-0
+    // to the sums, the same bits as skipping it while g is finite.
+    float wxm[3], wym[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      wxm[a] = ix[a] ? wx[a] : 0.0f;
+      wym[a] = iy[a] ? wy[a] : 0.0f;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int r = 0; r < 3; r++) wxy[a][r] = wxm[r] * wym[a];
+    // bases past a far face (keyed into the last block) have zero weights
+    // on that axis; their tile offset is clamped into the tile
+    const int ox = min(st.g[0] - gmeta[slot][0], 3), oy = min(st.g[1] - gmeta[slot][1], 3),
+              oz = min(st.g[2] - gmeta[slot][2], 3);
+    const float4* tb = gtile + slot * kTileNodes + ox + kTileSide * oy + kTileSide * kTileSide * oz;
+#pragma unroll 1
+    for (int dk = 0; dk < 3; dk++) {
+      const float wzk = iz[dk] ? wz[dk] : 0.0f;
+      const float4* tk = tb + dk * (kTileSide * kTileSide);
+      float t[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int dj = 0; dj < 3; dj++)
+#pragma unroll
+        for (int di = 0; di < 3; di++) {
+          const float w0 = wxy[dj][di] * wzk;
+          const float4 g = tk[di + kTileSide * dj];
+          const float w = !(w0 < 1e-10f) ? w0 : 0.0f;
+          const float u[3] = {w * g.x, w * g.y, w * g.z};
+#pragma unroll
+          for (int r = 0; r < 3; r++) {
+            sx[di][r] += u[r];
+            sy[dj][r] += u[r];
+            t[r] += u[r];
+          }
+        }
+      layer_sums(dk, t);
+    }
+  } else {
+#pragma unroll 1
+    for (int dk = 0; dk < 3; dk++) {
+      const float wzk = wz[dk];
+      float t[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int dj = 0; dj < 3; dj++)
+#pragma unroll
+        for (int di = 0; di < 3; di++) {
+          const float w0 = wxy[dj][di] * wzk;
+          const float4 g = hbm_node(di, dj, dk);
+          // bitwise &: every lane loads the whole node (no load behind a branch)
+          const bool keep = !(w0 < 1e-10f) & ix[di] & iy[dj] & iz[dk] & !(g.w < 1e-12f);
+          const float u[3] = {keep ? w0 * g.x : 0.0f, keep ? w0 * g.y : 0.0f,
+                              keep ? w0 * g.z : 0.0f};
+#pragma unroll
+          for (int r = 0; r < 3; r++) {
+            sx[di][r] += u[r];
+            sy[dj][r] += u[r];
+            t[r] += u[r];
+          }
+        }
+      layer_sums(dk, t);
+    }
+  }
 #pragma unroll
   for (int r = 0; r < 3; r++) {
     nc[3 * r + 0] = k4 * (sx[0][r] * dxa[0] + sx[1][r] * dxa[1] + sx[2][r] * dxa[2]);
