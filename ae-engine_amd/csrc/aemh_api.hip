@@ -31,6 +31,12 @@ namespace {
 
 const char* kVersion = "aemh 0.1 (gfx950)";
 
+// Substeps between the G2P passes that re-sort the working set by block
+// (the others update it in place).
+constexpr uint32_t kResortEvery = 8;
+// Working-set slots are addressed with 32-bit byte offsets in the kernels.
+constexpr uint32_t kMaxWork = 1u << 30;
+
 struct ParticleStore {
   uint32_t cap = 0;
   int storage = 0;
@@ -120,9 +126,12 @@ struct aemh_ctx {
   LaunchShape shape;
   bool tuned = false;
   float tune_ms[3] = {0.0f, 0.0f, 0.0f};  // best P2G, grid update, G2P time
-  // graphs
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  // graphs: one per (ping-pong parity, re-sort step), built on first use
+  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
   bool graph_valid = false;
+  // in-place substeps between re-sorting ones
+  uint32_t since_sort = 0;
+  bool need_sort = true;
   bool use_graph = true;
   // profiling
   bool prof = false;
@@ -162,21 +171,45 @@ aemh_status fail(aemh_ctx* c, aemh_status s, const char* fmt, ...) {
                   hipGetErrorString(e_));                                                    \
   } while (0)
 
+// Per-slot arrays are carved from one slab, 256 B aligned, each followed by
+// an odd multiple of 256 B: with power-of-two particle counts the arrays
+// would otherwise start power-of-two bytes apart, and the ~26 streams a
+// kernel reads or writes at the same slot would meet in the same HBM
+// channels.
+constexpr size_t kPlanePad = 9 * 256;
+
 template <typename T>
 T* carve(char*& p, size_t count) {
   T* r = reinterpret_cast<T*>(p);
   size_t bytes = (count * sizeof(T) + 255) & ~(size_t)255;
-  p += bytes;
+  p += bytes + kPlanePad;
   return r;
 }
 
+void carve_store(char* p, PBuf& pb, uint32_t n, int storage) {
+  for (int a = 0; a < 3; a++) pb.x[a] = carve<float>(p, n);
+  for (int a = 0; a < 3; a++) pb.v[a] = carve<float>(p, n);
+  for (int k = 0; k < 9; k++) {
+    if (storage == 0) {
+      pb.C[k] = carve<float>(p, n);
+      pb.F[k] = carve<float>(p, n);
+    } else {
+      pb.Ch[k] = carve<uint16_t>(p, n);
+      pb.Fh[k] = carve<uint16_t>(p, n);
+    }
+  }
+  pb.J = carve<float>(p, n);
+  pb.Jp = carve<float>(p, n);
+  pb.mass = carve<float>(p, n);
+  pb.mat = carve<uint16_t>(p, n);
+  pb.id = carve<uint32_t>(p, n);
+}
+
 size_t store_bytes(uint32_t n, int storage) {
-  auto a = [&](size_t elem) { return ((size_t)n * elem + 255) & ~(size_t)255; };
-  size_t b = 6 * a(4);                          // x, v
-  b += storage == 0 ? 18 * a(4) : 18 * a(2);   // C, F
-  b += 3 * a(4);                                // J, Jp, mass
-  b += a(2) + a(4);                             // mat, id
-  return b;
+  PBuf pb{};
+  char* base = reinterpret_cast<char*>((uintptr_t)4096);  // layout only, never dereferenced
+  carve_store(base, pb, n, storage);
+  return (size_t)((char*)pb.id - base) + (((size_t)n * 4 + 255) & ~(size_t)255) + kPlanePad;
 }
 
 aemh_status alloc_store(aemh_ctx* c, ParticleStore& s, uint32_t n, int storage) {
@@ -190,23 +223,7 @@ aemh_status alloc_store(aemh_ctx* c, ParticleStore& s, uint32_t n, int storage) 
   s.pb = PBuf{};
   if (n == 0) return AEMH_OK;
   HIPTRY(c, hipMalloc(&s.slab, store_bytes(n, storage)));
-  char* p = (char*)s.slab;
-  for (int a = 0; a < 3; a++) s.pb.x[a] = carve<float>(p, n);
-  for (int a = 0; a < 3; a++) s.pb.v[a] = carve<float>(p, n);
-  for (int k = 0; k < 9; k++) {
-    if (storage == 0) {
-      s.pb.C[k] = carve<float>(p, n);
-      s.pb.F[k] = carve<float>(p, n);
-    } else {
-      s.pb.Ch[k] = carve<uint16_t>(p, n);
-      s.pb.Fh[k] = carve<uint16_t>(p, n);
-    }
-  }
-  s.pb.J = carve<float>(p, n);
-  s.pb.Jp = carve<float>(p, n);
-  s.pb.mass = carve<float>(p, n);
-  s.pb.mat = carve<uint16_t>(p, n);
-  s.pb.id = carve<uint32_t>(p, n);
+  carve_store((char*)s.slab, s.pb, n, storage);
   return AEMH_OK;
 }
 
@@ -221,7 +238,7 @@ aemh_status ensure_staging(aemh_ctx* c, size_t bytes) {
 }
 
 void invalidate_graphs(aemh_ctx* c) {
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < 4; i++)
     if (c->gexec[i]) {
       hipGraphExecDestroy(c->gexec[i]);
       c->gexec[i] = nullptr;
@@ -549,8 +566,11 @@ aemh_status mpss_ready(aemh_ctx* c) {
 }
 
 // Launch one substep on the context stream (eager or under capture).
-void enqueue_substep(aemh_ctx* c, int cur) {
-  const PBuf a = c->work[cur].pb, b = c->work[cur ^ 1].pb;
+// One substep on work[cur].  A re-sort step writes G2P's result to
+// work[cur ^ 1] in block order; any other step updates work[cur] in place.
+void enqueue_substep(aemh_ctx* c, int cur, bool resort) {
+  const PBuf a = c->work[cur].pb, b = resort ? c->work[cur ^ 1].pb : a;
+  const int inplace = resort ? 0 : 1;
   const uint32_t n = c->n_work;
   const int ns = (int)c->scd_h.size();
   const int storage = c->work[0].storage;
@@ -567,7 +587,7 @@ void enqueue_substep(aemh_ctx* c, int cur) {
     });
     timed(c, "grid_update", [&] { det::launch_grid_update(st, c->shape, c->scd_d, ns, c->sort, c->grid, n, c->nb_total); });
     timed(c, "g2p", [&] {
-      det::launch_g2p(st, c->shape, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
+      det::launch_g2p(st, c->shape, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, inplace);
     });
   } else {
     timed(c, "p2g", [&] {
@@ -575,23 +595,20 @@ void enqueue_substep(aemh_ctx* c, int cur) {
     });
     timed(c, "grid_update", [&] { fast::launch_grid_update(st, c->shape, c->scd_d, ns, c->sort, c->grid, n, c->nb_total); });
     timed(c, "g2p", [&] {
-      fast::launch_g2p(st, c->shape, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
+      fast::launch_g2p(st, c->shape, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, inplace);
     });
   }
 }
 
-aemh_status build_graphs(aemh_ctx* c) {
-  invalidate_graphs(c);
-  for (int par = 0; par < 2; par++) {
-    hipGraph_t g = nullptr;
-    HIPTRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_substep(c, par);
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (e != hipSuccess) return fail(c, AEMH_ERR_HIP, "graph capture: %s", hipGetErrorName(e));
-    e = hipGraphInstantiate(&c->gexec[par], g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    if (e != hipSuccess) return fail(c, AEMH_ERR_HIP, "graph instantiate: %s", hipGetErrorName(e));
-  }
+aemh_status build_graph(aemh_ctx* c, int cur, bool resort) {
+  hipGraph_t g = nullptr;
+  HIPTRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_substep(c, cur, resort);
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (e != hipSuccess) return fail(c, AEMH_ERR_HIP, "graph capture: %s", hipGetErrorName(e));
+  e = hipGraphInstantiate(&c->gexec[cur * 2 + (resort ? 1 : 0)], g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(c, AEMH_ERR_HIP, "graph instantiate: %s", hipGetErrorName(e));
   c->graph_valid = true;
   return AEMH_OK;
 }
@@ -624,18 +641,28 @@ aemh_status run_substeps(aemh_ctx* c, uint32_t k, bool allow_graph = true) {
   // eagerly with an event pair around each kernel section: event-record
   // nodes inside a graph cost the replay more than eager launches do.
   const bool graph = allow_graph && c->use_graph && !c->prof;
-  if (graph && !c->graph_valid) {
-    aemh_status s = build_graphs(c);
-    if (s != AEMH_OK) return s;
-  }
   for (uint32_t i = 0; i < k; i++) {
+    // Re-sort on the first substep of a new working set, then every
+    // kResortEvery substeps; the others update the particles in place.
+    const bool resort = c->need_sort || c->since_sort + 1 >= kResortEvery;
     if (graph) {
-      HIPTRY(c, hipGraphLaunch(c->gexec[c->cur], c->stream));
+      hipGraphExec_t& ge = c->gexec[c->cur * 2 + (resort ? 1 : 0)];
+      if (!ge) {
+        aemh_status s = build_graph(c, c->cur, resort);
+        if (s != AEMH_OK) return s;
+      }
+      HIPTRY(c, hipGraphLaunch(ge, c->stream));
     } else {
-      enqueue_substep(c, c->cur);
+      enqueue_substep(c, c->cur, resort);
       CHECK_LAUNCH(c);
     }
-    c->cur ^= 1;
+    if (resort) {
+      c->cur ^= 1;
+      c->since_sort = 0;
+      c->need_sort = false;
+    } else {
+      c->since_sort += 1;
+    }
   }
   c->work_dirty = true;
   HIPTRY(c, hipStreamSynchronize(c->stream));
@@ -839,6 +866,7 @@ aemh_status tier_set_work(aemh_ctx* c, uint64_t key, const uint32_t* const* list
   if (s != AEMH_OK) return s;
   uint32_t n = 0;
   for (int k = 0; k < nl; k++) n += counts[k];
+  if (n >= kMaxWork) return fail(c, AEMH_ERR_ARG, "working set of %u particles: limit 2^30 - 1", n);
   const int storage = (int)c->cfg.storage;
   for (int k = 0; k < 2; k++)
     if ((s = alloc_store(c, c->work[k], n ? n : 1, storage)) != AEMH_OK) return s;
@@ -855,6 +883,7 @@ aemh_status tier_set_work(aemh_ctx* c, uint64_t key, const uint32_t* const* list
   HIPTRY(c, hipStreamSynchronize(c->stream));
   c->n_work = n;
   c->cur = 0;
+  c->need_sort = true;  // list order: the first substep re-sorts
   c->idx_h.clear();
   c->idx_explicit = false;
   c->work_default = false;
@@ -1082,6 +1111,7 @@ aemh_status aemh_set_indices(aemh_ctx* c, const uint32_t* idx, uint32_t n_idx) {
   if (!c) return AEMH_ERR_ARG;
   if (!c->canon.slab) return fail(c, AEMH_ERR_STATE, "no particles uploaded");
   if (n_idx && !idx) return fail(c, AEMH_ERR_ARG, "null index list");
+  if (n_idx >= kMaxWork) return fail(c, AEMH_ERR_ARG, "working set of %u particles: limit 2^30 - 1", n_idx);
   hipSetDevice(c->device);
   // Distinct, in-range indices (the reference relies on this by convention,
   // mpm_solver.rs:1462; a duplicate would make two G2P writers race here).
@@ -1105,6 +1135,7 @@ aemh_status aemh_set_indices(aemh_ctx* c, const uint32_t* idx, uint32_t n_idx) {
   c->work_default = false;
   c->n_work = n_idx;
   c->cur = 0;
+  c->need_sort = true;  // list order: the first substep re-sorts
   launch_gather_work(c->stream, c->canon.pb, c->work[0].pb, c->idx_d, n_idx, storage);
   CHECK_LAUNCH(c);
   HIPTRY(c, hipStreamSynchronize(c->stream));
@@ -1180,7 +1211,8 @@ aemh_status aemh_batch_step(aemh_ctx* c, uint32_t n_substeps) {
 // flush mode, with the grid update that consumes it), the grid update
 // (workgroup count) and G2P (workgroup size) on the current particles and
 // scenes and keeps the fastest of each.  The substep kernels
-// run without advancing the state: G2P writes the idle ping-pong buffer and
+// run without advancing the state: G2P, timed in its in-place form, writes
+// the idle ping-pong buffer (which the next re-sorting step overwrites) and
 // the sort and grid buffers are rebuilt by the next step.
 aemh_status aemh_autotune(aemh_ctx* c) {
   if (!c) return AEMH_ERR_ARG;
@@ -1254,13 +1286,13 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   c->tune_ms[1] = best;
   best = 1e30f;
   pick = ls.g2p_wg;
-  for (int wg : {64, 128, 256}) {
+  for (int wg : {64, 128, 256, 512}) {
     ls.g2p_wg = wg;
     s = time_launches(c, kReps, &ms, [&] {
       if (q32)
-        det::launch_g2p(st, ls, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
+        det::launch_g2p(st, ls, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, 1);
       else
-        fast::launch_g2p(st, ls, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
+        fast::launch_g2p(st, ls, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, 1);
     });
     if (s != AEMH_OK) return s;
     if (ms < best) best = ms, pick = wg;
@@ -1294,8 +1326,8 @@ aemh_status aemh_set_launch_shape(aemh_ctx* c, uint32_t p2g_tiles, uint32_t p2g_
   if (!c) return AEMH_ERR_ARG;
   if (p2g_tiles < 2 || p2g_tiles > 4) return fail(c, AEMH_ERR_ARG, "p2g_tiles must be 2, 3 or 4");
   if (p2g_partials > 1) return fail(c, AEMH_ERR_ARG, "p2g_partials must be 0 or 1");
-  if (g2p_wg != 64 && g2p_wg != 128 && g2p_wg != 256)
-    return fail(c, AEMH_ERR_ARG, "g2p_wg must be 64, 128 or 256");
+  if (g2p_wg != 64 && g2p_wg != 128 && g2p_wg != 256 && g2p_wg != 512)
+    return fail(c, AEMH_ERR_ARG, "g2p_wg must be 64, 128, 256 or 512");
   if (upd_wgs < 1 || upd_wgs > 65536) return fail(c, AEMH_ERR_ARG, "upd_wgs must be in [1, 65536]");
   c->shape.p2g_tiles = (int)p2g_tiles;
   c->shape.p2g_partials = (int)p2g_partials;

@@ -233,6 +233,52 @@ __device__ __forceinline__ float get_f(const PBuf& P, uint32_t j, int k, int sto
   float d = storage == 1 ? bf16_to_f32(P.Fh[k][j]) : f16_to_f32(P.Fh[k][j]);
   return d + ((k % 4 == 0) ? 1.0f : 0.0f);
 }
+// Slot addressing as uniform array base + 32-bit byte offset: the compiler
+// then emits global loads and stores with the base in SGPRs and one shared
+// VGPR offset, instead of a 64-bit address per access.  Working sets are
+// below 2^30 slots, so the offsets fit.
+struct Slot {
+  uint32_t b4, b2;  // byte offsets in 4-byte and 2-byte per-slot arrays
+};
+__device__ __forceinline__ Slot slot_of(uint32_t i) { return Slot{i << 2, i << 1}; }
+template <typename T>
+__device__ __forceinline__ T& at4(T* base, const Slot& s) {
+  return *(T*)((char*)base + s.b4);
+}
+template <typename T>
+__device__ __forceinline__ T& at2(T* base, const Slot& s) {
+  return *(T*)((char*)base + s.b2);
+}
+__device__ __forceinline__ void put_c(PBuf& P, const Slot& s, int k, float c, int storage) {
+  if (storage == 0) at4(P.C[k], s) = c;
+  else at2(P.Ch[k], s) = storage == 1 ? f32_to_bf16(c) : f32_to_f16(c);
+}
+__device__ __forceinline__ void put_f(PBuf& P, const Slot& s, int k, float f, int storage) {
+  if (storage == 0) at4(P.F[k], s) = f;
+  else {
+    float d = f - ((k % 4 == 0) ? 1.0f : 0.0f);
+    at2(P.Fh[k], s) = storage == 1 ? f32_to_bf16(d) : f32_to_f16(d);
+  }
+}
+__device__ __forceinline__ float get_f(const PBuf& P, const Slot& s, int k, int storage) {
+  if (storage == 0) return at4(P.F[k], s);
+  const uint16_t h = at2(P.Fh[k], s);
+  float d = storage == 1 ? bf16_to_f32(h) : f16_to_f32(h);
+  return d + ((k % 4 == 0) ? 1.0f : 0.0f);
+}
+__device__ __forceinline__ void copy_cf(PBuf& D, const Slot& o, const PBuf& S, const Slot& p,
+                                        int storage) {
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    if (storage == 0) {
+      at4(D.C[k], o) = at4(S.C[k], p);
+      at4(D.F[k], o) = at4(S.F[k], p);
+    } else {
+      at2(D.Ch[k], o) = at2(S.Ch[k], p);
+      at2(D.Fh[k], o) = at2(S.Fh[k], p);
+    }
+  }
+}
 // C and F unchanged from slot p of S to slot j of D, copied in their stored
 // form (no 16-bit round trip through FP32).
 __device__ __forceinline__ void copy_cf(PBuf& D, uint32_t j, const PBuf& S, uint32_t p, int storage) {

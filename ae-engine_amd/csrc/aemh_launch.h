@@ -15,7 +15,7 @@ struct LaunchShape {
   int p2g_tiles = 3;     // LDS tiles per P2G workgroup: 2, 3 or 4
   int p2g_partials = 0;  // P2G tile flush: 0 one HBM atomic per node value,
                          // 1 partial stores gathered by the grid update
-  int g2p_wg = 256;      // G2P workgroup size: 64, 128 or 256
+  int g2p_wg = 256;      // G2P workgroup size: 64, 128, 256 or 512
   int upd_wgs = 2048;    // persistent grid-update / clear workgroups
 };
 
@@ -86,7 +86,7 @@ void launch_gather_pos(hipStream_t st, PBuf P, const uint32_t* list, uint32_t n,
                           int ns, SortDev s, GridDev g, uint32_t n, uint32_t nb_total);     \
   void launch_g2p(hipStream_t st, const LaunchShape& ls, PBuf cur, PBuf nxt,                \
                   const SceneDev* sc, int ns, const MatDev* mats, int nmat, uint32_t n,     \
-                  SortDev s, GridDev g, int storage);                                       \
+                  SortDev s, GridDev g, int storage, int inplace);                          \
   }
 AEMH_DECLARE_MODE(fast)
 AEMH_DECLARE_MODE(det)

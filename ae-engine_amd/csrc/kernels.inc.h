@@ -41,7 +41,13 @@ constexpr int kPartStride = 8;
 // Lane layers of the P2G re-assignment (rank within a base cell, capped).
 constexpr int kLayers = 16;
 // Base-block tiles of grid (v, m) a G2P workgroup stages in LDS.
-constexpr int kG2PTiles = 4;
+// G2P stages up to two 6^3 grid tiles per wave of the workgroup in LDS
+// (dynamic shared memory, 3,456 B per tile), workgroups of up to 512.
+constexpr int kG2PMaxWG = 512;
+constexpr int kG2PMaxTiles = 16;
+__host__ __device__ constexpr int g2p_tiles_for(int wg) {
+  return (2 * (wg >> 6)) < kG2PMaxTiles ? 2 * (wg >> 6) : kG2PMaxTiles;
+}
 
 // Per-particle P2G inputs after the constitutive update.
 struct P2GPart {
@@ -619,21 +625,29 @@ __device__ __forceinline__ void node_axes(uint32_t boff, int nb0, int nb1, int x
   az = ((boff + (uint32_t)((z >> 2) * nb1 * nb0)) << 6) | ((uint32_t)(z & 3) << 4);
 }
 
-// G2P: sorted position j reads working slot perm[j] of `cur` and writes slot j
-// of `nxt`, so the particle arrays come out ordered by base block.
+// G2P.  Re-sort step (inplace = 0): sorted position j advances working slot
+// p = perm[j] of `cur` and writes slot j of `nxt` with the particle's mass,
+// material and id, so the arrays come out ordered by base block.  In-place
+// step (inplace = 1): thread j advances slot j and writes it back to slot j
+// of `nxt` (`cur` itself, or a scratch copy for timing), leaving the fields
+// G2P does not change; particles cross blocks rarely, so between re-sorts a
+// workgroup's slots still fall in a few blocks.
 template <int STORE>
-__global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev* __restrict__ sc,
+__global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const SceneDev* __restrict__ sc,
                                              int ns, const MatDev* __restrict__ mats, int nmat,
-                                             uint32_t n, SortDev s, GridDev G) {
-  // The grid nodes of the workgroup's first kG2PTiles base blocks are staged
-  // in LDS (one 6^3 tile each, loaded in one coalesced pass), so the 27-node
-  // gather reads LDS instead of waiting on three dependent rounds of global
-  // loads; particles of further blocks gather from global memory.
-  __shared__ float4 gtile[kG2PTiles * kTileNodes];
-  __shared__ uint32_t skey[256];
-  __shared__ int gmeta[kG2PTiles][8];  // origin x, y, z; grid n x, y, z; nb0; nb1
-  __shared__ uint32_t gboff[kG2PTiles];
-  __shared__ uint32_t wcnt[4];
+                                             uint32_t n, SortDev s, GridDev G, int inplace) {
+  // The grid nodes of the workgroup's first g2p_tiles_for(blockDim) base
+  // blocks are staged in LDS (one 6^3 tile each, loaded in one coalesced
+  // pass), so the 27-node gather reads LDS instead of waiting on three
+  // dependent rounds of global loads; particles of further blocks gather
+  // from global memory.  Larger workgroups load fewer tiles per particle: the
+  // particles of one block are split over fewer workgroups.
+  extern __shared__ float4 gtile[];
+  __shared__ uint32_t skey[kG2PMaxWG];
+  __shared__ int gmeta[kG2PMaxTiles][8];  // origin x, y, z; grid n x, y, z; nb0; nb1
+  __shared__ uint32_t gboff[kG2PMaxTiles];
+  __shared__ uint32_t wcnt[kG2PMaxWG / 64];
+  const int max_tiles = g2p_tiles_for((int)blockDim.x);
   const int t = threadIdx.x;
   const int lane = __lane_id(), wave = t >> 6;
   const uint32_t j = blockIdx.x * blockDim.x + (uint32_t)t;
@@ -643,24 +657,28 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
   // overlap the tile staging below instead of following its barriers.  The
   // block key comes from the position (the count kernel's function of the
   // same x), not from a second dependent load.
-  const uint32_t p = valid ? s.perm[j] : 0u;
+  // An in-place step walks the slots themselves: particles are independent
+  // here, and the working set has stayed nearly in block order since the
+  // last re-sort, so the workgroup's slots still share a few tiles.
+  const uint32_t p = valid ? (inplace ? j : s.perm[j]) : 0u;
   float px = 0.0f, py = 0.0f, pz = 0.0f, pmass = 0.0f, pjp = 1.0f;
   uint16_t mi = 0;
   uint32_t pid = 0;
   float F[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) F[k] = (k % 4 == 0) ? 1.0f : 0.0f;
+  const Slot ps = slot_of(p);
   if (valid) {
-    px = cur.x[0][p];
-    py = cur.x[1][p];
-    pz = cur.x[2][p];
-    mi = cur.mat[p];
-    pmass = cur.mass[p];
-    pid = cur.id[p];
-    pjp = cur.Jp[p];
+    px = at4(cur.x[0], ps);
+    py = at4(cur.x[1], ps);
+    pz = at4(cur.x[2], ps);
+    mi = at2(cur.mat, ps);
+    pmass = at4(cur.mass, ps);
+    pid = at4(cur.id, ps);
+    pjp = at4(cur.Jp, ps);
     if (!S.vel_only) {
 #pragma unroll
-      for (int k = 0; k < 9; k++) F[k] = get_f(cur, p, k, STORE);
+      for (int k = 0; k < 9; k++) F[k] = get_f(cur, ps, k, STORE);
     }
   }
   const Stencil st = make_stencil(S, px, py, pz);
@@ -678,8 +696,8 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
     tot += wcnt[w];
   }
   const int slot = (int)(pre + (uint32_t)__popcll(hb & le)) - 1;
-  const int ntiles = (int)min(tot, (uint32_t)kG2PTiles);
-  if (head && slot < kG2PTiles) {
+  const int ntiles = (int)min(tot, (uint32_t)max_tiles);
+  if (head && slot < max_tiles) {
     const uint32_t lb = key - S.block_off;
     gmeta[slot][0] = (int)(lb % (uint32_t)S.nb[0]) * 4;
     gmeta[slot][1] = (int)((lb / (uint32_t)S.nb[0]) % (uint32_t)S.nb[1]) * 4;
@@ -712,7 +730,7 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
   }
   __syncthreads();
   if (!valid) return;
-  const bool use_lds = slot < kG2PTiles;
+  const bool use_lds = slot < max_tiles;
   const float4* __restrict__ gvp = G.gv;
   const int n0 = S.n[0], n1 = S.n[1], n2 = S.n[2];
   const int nb0 = S.nb[0], nb1 = S.nb[1];
@@ -758,9 +776,12 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
   auto lds_node = [&](int di, int dj, int dk) -> float4 { return gtile[tz[dk] + ty[dj] + tx[di]]; };
   auto hbm_node = [&](int di, int dj, int dk) -> float4 { return gvp[az[dk] + ay[dj] + ax[di]]; };
   const float dt = S.dt;
-  nxt.mass[j] = pmass;
-  nxt.mat[j] = mi;
-  nxt.id[j] = pid;
+  const Slot os = slot_of(inplace ? p : j);  // output slot
+  if (!inplace) {
+    at4(nxt.mass, os) = pmass;
+    at2(nxt.mat, os) = mi;
+    at4(nxt.id, os) = pid;
+  }
 
   if (S.vel_only) {  // mpm_solver.rs:1465-1571
     float nv0 = 0.0f, nv1 = 0.0f, nv2 = 0.0f;
@@ -814,15 +835,17 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
       x2 = S.gmax_z + S.o[2];
       nv2 = fminf(nv2, 0.0f) * 0.5f;
     }
-    nxt.x[0][j] = x0;
-    nxt.x[1][j] = x1;
-    nxt.x[2][j] = x2;
-    nxt.v[0][j] = nv0 * S.damping;
-    nxt.v[1][j] = nv1 * S.damping;
-    nxt.v[2][j] = nv2 * S.damping;
-    copy_cf(nxt, j, cur, p, STORE);
-    nxt.J[j] = cur.J[p];
-    nxt.Jp[j] = pjp;
+    at4(nxt.x[0], os) = x0;
+    at4(nxt.x[1], os) = x1;
+    at4(nxt.x[2], os) = x2;
+    at4(nxt.v[0], os) = nv0 * S.damping;
+    at4(nxt.v[1], os) = nv1 * S.damping;
+    at4(nxt.v[2], os) = nv2 * S.damping;
+    if (!inplace) {  // C, F, J, J_p are unchanged by this tier
+      copy_cf(nxt, os, cur, ps, STORE);
+      at4(nxt.J, os) = at4(cur.J, ps);
+      at4(nxt.Jp, os) = pjp;
+    }
     return;
   }
 
@@ -924,13 +947,16 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
     const int ox = min(st.g[0] - gmeta[slot][0], 3), oy = min(st.g[1] - gmeta[slot][1], 3),
               oz = min(st.g[2] - gmeta[slot][2], 3);
     const float4* tb = gtile + slot * kTileNodes + ox + kTileSide * oy + kTileSide * kTileSide * oz;
+    // Per row of 3 nodes (fixed dj, dk) the row sum feeds the y-axis sums
+    // and the layer sum once, instead of every node adding to all three.
 #pragma unroll 1
     for (int dk = 0; dk < 3; dk++) {
       const float wzk = iz[dk] ? wz[dk] : 0.0f;
       const float4* tk = tb + dk * (kTileSide * kTileSide);
-      float t[3] = {0.0f, 0.0f, 0.0f};
+      float t[3];
 #pragma unroll
-      for (int dj = 0; dj < 3; dj++)
+      for (int dj = 0; dj < 3; dj++) {
+        float row[3];
 #pragma unroll
         for (int di = 0; di < 3; di++) {
           const float w0 = wxy[dj][di] * wzk;
@@ -940,10 +966,15 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
 #pragma unroll
           for (int r = 0; r < 3; r++) {
             sx[di][r] += u[r];
-            sy[dj][r] += u[r];
-            t[r] += u[r];
+            row[r] = di == 0 ? u[r] : row[r] + u[r];
           }
         }
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          sy[dj][r] += row[r];
+          t[r] = dj == 0 ? row[r] : t[r] + row[r];
+        }
+      }
       layer_sums(dk, t);
     }
   } else {
@@ -1024,14 +1055,14 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
   }
   // x, v and C are final here: store them before the F update so they are
   // not live across the snow SVD.
-  nxt.x[0][j] = np[0];
-  nxt.x[1][j] = np[1];
-  nxt.x[2][j] = np[2];
-  nxt.v[0][j] = nv[0] * S.damping;
-  nxt.v[1][j] = nv[1] * S.damping;
-  nxt.v[2][j] = nv[2] * S.damping;
+  at4(nxt.x[0], os) = np[0];
+  at4(nxt.x[1], os) = np[1];
+  at4(nxt.x[2], os) = np[2];
+  at4(nxt.v[0], os) = nv[0] * S.damping;
+  at4(nxt.v[1], os) = nv[1] * S.damping;
+  at4(nxt.v[2], os) = nv[2] * S.damping;
 #pragma unroll
-  for (int k = 0; k < 9; k++) put_c(nxt, j, k, nc[k], STORE);
+  for (int k = 0; k < 9; k++) put_c(nxt, os, k, nc[k], STORE);
   // Q9: F' = (I + dt grad v) F; validity; J = clamp(det F', 0.1, 10).
   float nf[9];
 #pragma unroll
@@ -1064,9 +1095,9 @@ __global__ __launch_bounds__(256) void k_g2p(PBuf cur, PBuf nxt, const SceneDev*
     fj = 1.0f;
   }
 #pragma unroll
-  for (int k = 0; k < 9; k++) put_f(nxt, j, k, nf[k], STORE);
-  nxt.J[j] = fj;
-  nxt.Jp[j] = jp;
+  for (int k = 0; k < 9; k++) put_f(nxt, os, k, nf[k], STORE);
+  at4(nxt.J, os) = fj;
+  at4(nxt.Jp, os) = jp;
 }
 
 }  // namespace
@@ -1103,16 +1134,17 @@ void launch_grid_update(hipStream_t st, const LaunchShape& ls, const SceneDev* s
 
 void launch_g2p(hipStream_t st, const LaunchShape& ls, PBuf cur, PBuf nxt, const SceneDev* sc,
                 int ns, const MatDev* mats, int nmat, uint32_t n, SortDev s, GridDev g,
-                int storage) {
+                int storage, int inplace) {
   if (!n) return;
   const unsigned wg = (unsigned)ls.g2p_wg;
   const unsigned grid = (n + wg - 1u) / wg;
+  const size_t lds = (size_t)g2p_tiles_for((int)wg) * kTileNodes * sizeof(float4);
   if (storage == 1)
-    k_g2p<1><<<grid, wg, 0, st>>>(cur, nxt, sc, ns, mats, nmat, n, s, g);
+    k_g2p<1><<<grid, wg, lds, st>>>(cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
   else if (storage == 2)
-    k_g2p<2><<<grid, wg, 0, st>>>(cur, nxt, sc, ns, mats, nmat, n, s, g);
+    k_g2p<2><<<grid, wg, lds, st>>>(cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
   else
-    k_g2p<0><<<grid, wg, 0, st>>>(cur, nxt, sc, ns, mats, nmat, n, s, g);
+    k_g2p<0><<<grid, wg, lds, st>>>(cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
 }
 
 }  // namespace AEMH_NS

@@ -57,7 +57,7 @@ def sparse_snow():
 
 
 @pytest.mark.parametrize("shape", [(2, 0, 64, 1024), (3, 1, 128, 2048), (4, 1, 256, 4096),
-                                   (2, 1, 256, 16384), (4, 0, 128, 8192)])
+                                   (2, 1, 256, 16384), (4, 0, 128, 8192), (3, 0, 512, 2048)])
 def test_launch_shapes_deterministic_bitexact(sparse_snow, shape):
     wl, st = sparse_snow
     eng = aemh.Engine()
@@ -81,7 +81,7 @@ def test_autotune_does_not_advance_state():
     shape = eng.autotune()
     assert shape["p2g_tiles"] in (2, 3, 4)
     assert shape["p2g_partials"] in (0, 1)
-    assert shape["g2p_wg"] in (64, 128, 256)
+    assert shape["g2p_wg"] in (64, 128, 256, 512)
     assert shape["upd_wgs"] in (1024, 2048, 4096, 8192, 16384)
     assert all(t > 0 for t in shape["tuned_ms"])
     for _ in range(5):
@@ -115,7 +115,7 @@ def test_profiled_steps_are_exact_and_timed():
 def test_launch_shape_rejects_bad_values():
     eng = aemh.Engine()
     for bad in [(1, 1, 256, 2048), (5, 1, 256, 2048), (3, 2, 256, 2048), (3, 1, 96, 2048),
-                (3, 1, 256, 0)]:
+                (3, 1, 1024, 2048), (3, 1, 256, 0)]:
         with pytest.raises(aemh.AemhError):
             eng.set_launch_shape(*bad)
     eng.close()
