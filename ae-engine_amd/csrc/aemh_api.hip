@@ -31,9 +31,16 @@ namespace {
 
 const char* kVersion = "aemh 0.1 (gfx950)";
 
-// Substeps between the G2P passes that re-sort the working set by block
-// (the others update it in place).
-constexpr uint32_t kResortEvery = 8;
+// G2P re-sorts the working set by block when the count kernel's wave runs
+// (one per change of block along the slots) have grown by a quarter over
+// the count right after the last re-sort, and at least every kResortMax
+// substeps; the other substeps update the particles in place.
+constexpr uint32_t kResortMax = 64;
+// When both copies of the working set fit in MI355X's 256 MB Infinity Cache
+// the re-sorting G2P's extra traffic (mass, material, id, perm) stays in
+// that cache, and every substep re-sorts; above it, in-place steps save HBM
+// traffic.
+constexpr size_t kInfinityCacheBytes = (size_t)256 << 20;
 // Working-set slots are addressed with 32-bit byte offsets in the kernels.
 constexpr uint32_t kMaxWork = 1u << 30;
 
@@ -132,6 +139,9 @@ struct aemh_ctx {
   // in-place substeps between re-sorting ones
   uint32_t since_sort = 0;
   bool need_sort = true;
+  bool fresh_sort = false;     // the last substep re-sorted
+  uint32_t runs_base = 0;      // wave runs counted right after a re-sort
+  uint32_t* runs_h = nullptr;  // host-mapped: wave runs of the last count
   bool use_graph = true;
   // profiling
   bool prof = false;
@@ -373,6 +383,11 @@ aemh_status ensure_grid_sort(aemh_ctx* c) {
   if (!c->sort.counters) {
     HIPTRY(c, hipMalloc(&c->sort.counters, 64));
     HIPTRY(c, hipMemset(c->sort.counters, 0, 64));
+    const size_t rb = (size_t)kRunLanes * kRunStride * 4;
+    HIPTRY(c, hipMalloc(&c->sort.runs, rb));
+    HIPTRY(c, hipMemset(c->sort.runs, 0, rb));
+    HIPTRY(c, hipHostGetDevicePointer((void**)&c->sort.runs_host, c->runs_h, 0));
+    invalidate_graphs(c);
   }
   if (n > c->sort_n_cap || !c->sort.key) {
     for (void* p : {(void*)c->sort.key, (void*)c->sort.rank, (void*)c->sort.perm})
@@ -641,10 +656,13 @@ aemh_status run_substeps(aemh_ctx* c, uint32_t k, bool allow_graph = true) {
   // eagerly with an event pair around each kernel section: event-record
   // nodes inside a graph cost the replay more than eager launches do.
   const bool graph = allow_graph && c->use_graph && !c->prof;
+  bool counted_sorted = false;  // the last substep's count saw a re-sorted set
   for (uint32_t i = 0; i < k; i++) {
-    // Re-sort on the first substep of a new working set, then every
-    // kResortEvery substeps; the others update the particles in place.
-    const bool resort = c->need_sort || c->since_sort + 1 >= kResortEvery;
+    // Re-sort on the first substep of a new working set, when the block
+    // order has decayed, and at least every kResortMax substeps.
+    const bool resort = c->need_sort || c->since_sort + 1 >= kResortMax ||
+                        2 * store_bytes(c->n_work, c->work[0].storage) <= kInfinityCacheBytes;
+    counted_sorted = c->fresh_sort;
     if (graph) {
       hipGraphExec_t& ge = c->gexec[c->cur * 2 + (resort ? 1 : 0)];
       if (!ge) {
@@ -663,9 +681,15 @@ aemh_status run_substeps(aemh_ctx* c, uint32_t k, bool allow_graph = true) {
     } else {
       c->since_sort += 1;
     }
+    c->fresh_sort = resort;
   }
   c->work_dirty = true;
   HIPTRY(c, hipStreamSynchronize(c->stream));
+  const uint32_t runs = *(volatile uint32_t*)c->runs_h;
+  if (counted_sorted)
+    c->runs_base = runs;
+  else if (c->runs_base && runs > c->runs_base + c->runs_base / 4)
+    c->need_sort = true;
   if (c->prof) collect_profile(c);
   return check_overflow(c);
 }
@@ -884,6 +908,7 @@ aemh_status tier_set_work(aemh_ctx* c, uint64_t key, const uint32_t* const* list
   c->n_work = n;
   c->cur = 0;
   c->need_sort = true;  // list order: the first substep re-sorts
+  c->runs_base = 0;
   c->idx_h.clear();
   c->idx_explicit = false;
   c->work_default = false;
@@ -933,10 +958,13 @@ aemh_status aemh_create(aemh_ctx** out, int device) {
   c->device = device;
   aemh_config_default(&c->cfg);
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
-      hipHostMalloc((void**)&c->flag_h, 8, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->flag_h, 8, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->runs_h, 64, hipHostMallocMapped | hipHostMallocCoherent) !=
+          hipSuccess) {
     delete c;
     return AEMH_ERR_HIP;
   }
+  *c->runs_h = 0;
   *out = c;
   return AEMH_OK;
 }
@@ -956,6 +984,7 @@ aemh_status aemh_destroy(aemh_ctx* c) {
                   (void*)c->grid.accq, (void*)c->grid.gv, (void*)c->sort.key, (void*)c->sort.rank,
                   (void*)c->sort.perm, (void*)c->sort.cnt, (void*)c->sort.off, (void*)c->sort.tsum,
                   (void*)c->sort.flag, (void*)c->sort.active, (void*)c->sort.counters,
+                  (void*)c->sort.runs,
                   (void*)c->sort.bslot, (void*)c->grid.partf, (void*)c->grid.partq,
                   (void*)c->life.lifetime, (void*)c->life.age, (void*)c->life.active,
                   (void*)c->life_tmp.lifetime, (void*)c->life_tmp.age, (void*)c->life_tmp.active,
@@ -966,6 +995,7 @@ aemh_status aemh_destroy(aemh_ctx* c) {
                   c->tier_sort_tmp})
     if (p) hipFree(p);
   if (c->flag_h) hipHostFree(c->flag_h);
+  if (c->runs_h) hipHostFree(c->runs_h);
   hipStreamDestroy(c->stream);
   delete c;
   return AEMH_OK;
@@ -1136,6 +1166,7 @@ aemh_status aemh_set_indices(aemh_ctx* c, const uint32_t* idx, uint32_t n_idx) {
   c->n_work = n_idx;
   c->cur = 0;
   c->need_sort = true;  // list order: the first substep re-sorts
+  c->runs_base = 0;
   launch_gather_work(c->stream, c->canon.pb, c->work[0].pb, c->idx_d, n_idx, storage);
   CHECK_LAUNCH(c);
   HIPTRY(c, hipStreamSynchronize(c->stream));

@@ -110,10 +110,16 @@ struct SortDev {
   uint32_t* flag;       // [NB] node block active this substep
   uint32_t* active;     // [NB] list of active node blocks
   uint32_t* counters;   // [8], see kCnt*
+  uint32_t* runs;       // [64 x 16] wave-run counts of the count kernel, one
+                        // per 64 B line (kRunLanes counters, stride kRunStride)
+  uint32_t* runs_host;  // device address of a host-mapped word: the runs of
+                        // the last count (particles' block-order disorder)
   uint32_t* bslot;      // [NB] tile slot of the block in the P2G workgroup
                         // holding its first particle (>= tiles: no tile)
 };
 
 constexpr int kScanTile = 1024;
+constexpr int kRunLanes = 64;
+constexpr int kRunStride = 16;
 
 }  // namespace aemh

@@ -13,6 +13,12 @@ __global__ void k_step_begin(SortDev s) {
   if (threadIdx.x < kCntOverflow) s.counters[threadIdx.x] = 0;
 }
 
+// Runs of equal keys seen by one wave (how far the working set is from block
+// order), spread over kRunLanes counters on separate 64 B lines.
+__device__ __forceinline__ void count_runs(SortDev& s, uint32_t wave_id, uint32_t runs) {
+  atomicAdd(&s.runs[(wave_id % (uint32_t)kRunLanes) * kRunStride], runs);
+}
+
 // Key + rank: every working slot computes the global block of its clamped
 // base cell; runs of equal keys inside a wave take one atomic on the block
 // counter, so nearly sorted input costs about one atomic per wave.
@@ -44,6 +50,7 @@ __global__ __launch_bounds__(256) void k_count(PBuf cur, const SceneDev* __restr
   const int h = 63 - __clzll((long long)mine);
   const uint32_t base = __shfl(old, h, 64);
   if (valid) s.rank[i] = base + (uint32_t)(lane - h);
+  if (lane == 0) count_runs(s, blockIdx.x * 4u + (threadIdx.x >> 6), (uint32_t)__popcll(heads));
 }
 
 // Grids with few node blocks (a coarse tier grid holding half a million
@@ -64,6 +71,7 @@ __global__ __launch_bounds__(256) void k_count_lds(PBuf cur, const SceneDev* __r
   const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
   uint32_t keys[kCountRounds], loc[kCountRounds];
   bool own[kCountRounds];
+  uint32_t nruns = 0;
 #pragma unroll
   for (int r = 0; r < kCountRounds; r++) {
     const uint32_t i = (blockIdx.x * kCountRounds + r) * 256u + threadIdx.x;
@@ -79,6 +87,7 @@ __global__ __launch_bounds__(256) void k_count_lds(PBuf cur, const SceneDev* __r
     const bool head = valid && (lane == 0 || prev != key);
     const unsigned long long heads = __ballot(head);
     const int nvalid = __popcll(__ballot(valid));
+    nruns += (uint32_t)__popcll(heads);
     uint32_t old = 0;
     if (head) {
       const unsigned long long above = heads & ~le;
@@ -92,6 +101,7 @@ __global__ __launch_bounds__(256) void k_count_lds(PBuf cur, const SceneDev* __r
     loc[r] = base + (uint32_t)(lane - h);
     own[r] = head && old == 0u;  // first adder of this bin in the workgroup
   }
+  if (lane == 0) count_runs(s, blockIdx.x * 4u + (threadIdx.x >> 6), nruns);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kCountRounds; r++)
@@ -153,6 +163,15 @@ __global__ __launch_bounds__(256) void k_scan_tsum(SortDev s, uint32_t ntiles) {
   __shared__ uint32_t wtot[4];
   __shared__ uint32_t carry_s;
   if (threadIdx.x == 0) carry_s = 0;
+  // The count kernel's wave runs: total to the host-mapped word, counters
+  // cleared for the next substep.
+  if (threadIdx.x < 64) {
+    uint32_t r = s.runs[threadIdx.x * kRunStride];
+    s.runs[threadIdx.x * kRunStride] = 0u;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) r += __shfl_down(r, d, 64);
+    if (threadIdx.x == 0 && s.runs_host) *(volatile uint32_t*)s.runs_host = r;
+  }
   __syncthreads();
   for (uint32_t c = 0; c < ntiles; c += 256) {
     const uint32_t i = c + threadIdx.x;
