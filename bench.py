@@ -42,10 +42,12 @@ HBM_PEAK_GBS = 8000.0
 #   16 (v, m); G2P reads 16 -> step 88 B; P2G 28 B; G2P 16 B.
 BYTES = {"p2g": (110, 28), "g2p": (156, 16), "grid_update": (0, 44), "step": (214, 88)}
 KERNELS = ["step_begin", "count", "scan", "scatter", "clear", "p2g", "grid_update", "g2p"]
-# HBM bytes per dispatch on the 1M-particle snow workload, from rocprofv3
-# FETCH_SIZE (x 2.000) and WRITE_SIZE (x 1.000) against a 256 MiB copy, as
-# recorded in profiles/c2_snow_1m_hbm_traffic.md for this build's kernels.
-TRAFFIC_C2 = {"p2g": 246.9e6, "g2p": 229.8e6, "grid_update": 140.1e6}
+# HBM bytes per dispatch from rocprofv3 FETCH_SIZE (x 2.000) and WRITE_SIZE
+# (x 1.000), calibrated against a 256 MiB copy, as recorded in
+# profiles/c2_snow_1m_hbm_traffic.md (1M-particle snow workload) and
+# profiles/c3_three_boxes_16m_g2p.md (16M particles, in-place G2P).
+TRAFFIC_C2 = {"p2g": 246.3e6, "g2p": 251.1e6, "grid_update": 142.7e6}
+TRAFFIC_C3 = {"p2g": 3017e6, "g2p": 3068e6}
 
 
 def log(msg):
@@ -129,7 +131,7 @@ def extra_c3(W, aemh, steps=10):
     el = timed_steps(eng, wl.dt, steps)
     eng.profile(True)
     eng.profile_reset()
-    timed_steps(eng, wl.dt, 3)
+    timed_steps(eng, wl.dt, 16)
     kern = {k: eng.profile_get(k)[0] for k in KERNELS}
     blocks, npart = eng.last_substep_info()
     eng.close()
@@ -142,8 +144,8 @@ def extra_c3(W, aemh, steps=10):
         "ms_per_step": el / steps * 1e3,
         "active_block_fraction": blocks / total_blocks,
         "step_hbm_frac": round(step_bytes / (el / steps) / 1e9 / HBM_PEAK_GBS, 4),
-        "g2p": roofline_of(kern, "g2p", npart, blocks),
-        "p2g": roofline_of(kern, "p2g", npart, blocks),
+        "g2p": roofline_of(kern, "g2p", npart, blocks, TRAFFIC_C3),
+        "p2g": roofline_of(kern, "p2g", npart, blocks, TRAFFIC_C3),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
         "launch_shape": shape,
     }
@@ -337,18 +339,19 @@ def tier_cpu_port(tw, aemh, lod, near, mid, far, dt):
                       "the tick cadences (classify 1/8, near 1/1, mid 1/12, lifetimes 2/1)"}
 
 
-def probe(W, aemh):
+def probe(W, aemh, which="c2"):
     """Counter-collection run.  The copy moves exactly 256 MiB in and 256 MiB
     out, so FETCH_SIZE / WRITE_SIZE of its dispatch give the KiB each counter
     reports per real KiB; the dispatches of the 5 steps after the autotuner
-    (the launch shape the benchmark uses) follow."""
+    (the launch shape the benchmark uses) follow, on the 1M-particle snow
+    workload (c2) or the 16M-particle three-box workload (c3)."""
     import torch
     x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
     for _ in range(3):
         y.copy_(x)
     torch.cuda.synchronize()
-    wl = W.c2_snow_dam()
+    wl = W.c3_three_boxes() if which == "c3" else W.c2_snow_dam()
     eng = aemh.Engine()
     W.load_engine(eng, wl)
     eng.step(wl.dt)
@@ -366,9 +369,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--probe", action="store_true",
+    ap.add_argument("--extras", default="q32,c3,c4,c5,tiers",
+                    help="comma list of the extra workloads to run (q32, c3, c4, c5, tiers)")
+    ap.add_argument("--probe", nargs="?", const="c2", choices=("c2", "c3"),
                     help="short run for hardware-counter collection: a 256 MiB device copy "
-                         "(calibrates FETCH_SIZE / WRITE_SIZE) then 5 steps of the main workload")
+                         "(calibrates FETCH_SIZE / WRITE_SIZE) then 5 steps of the main workload "
+                         "(c2, the default) or of the 16M-particle workload (c3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: every box here has one GPU and the MPM step does "
@@ -379,7 +385,7 @@ def main():
     from aemh import workloads as W
 
     if args.probe:
-        probe(W, aemh)
+        probe(W, aemh, args.probe)
         return
 
     log("[bench] main workload: C2 1M snow particles, 128^3")
@@ -412,17 +418,22 @@ def main():
     extra = {"profiled_window_particle_updates_per_s": wl.n * args.steps / el_prof,
              "g2p_roofline": roofline_of(kern, "g2p", npart, blocks, TRAFFIC_C2),
              "active_blocks": blocks}
-    if not args.no_extras:
+    want = set() if args.no_extras else {e.strip() for e in args.extras.split(",") if e.strip()}
+    if "q32" in want:
         log("[bench] deterministic Q32.32 mode")
         wq, elq, elq_g, kq, _, _, _ = run_main(W, aemh, max(5, args.steps // 5), 2,
                                                  accum=aemh.ACCUM_Q32_32)
         extra["q32_particle_updates_per_s"] = wq.n * max(5, args.steps // 5) / elq
+    if "c3" in want:
         log("[bench] C3 16M particles")
         extra["c3"] = extra_c3(W, aemh)
+    if "c4" in want:
         log("[bench] C4 batched vs plain")
         extra["c4"] = extra_batched(W, aemh)
+    if "c5" in want:
         log("[bench] C5 mixed precision")
         extra["c5"] = extra_c5(W, aemh)
+    if "tiers" in want:
         log("[bench] LOD tier tick at engine scale")
         extra["tiers"] = extra_tiers(W, aemh, with_cpu=not args.no_cpu)
     line["extra"] = extra
