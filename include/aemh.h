@@ -253,7 +253,7 @@ aemh_status aemh_batch_step(aemh_ctx* ctx, uint32_t n_substeps);
  * keeps the fastest of each: P2G LDS tiles per workgroup (2/3/4) together
  * with the tile flush (one HBM atomic per node value, or partial sums stored
  * per tile and added by the grid update), grid-update workgroups
- * (1024..16384) and the G2P workgroup (64/128/256).  The particle state is
+ * (1024..16384) and the G2P workgroup (64/128/256/512).  The particle state is
  * not advanced.  Called once by the first batch step; plain steps use the
  * defaults unless this is called.  No reference counterpart. */
 aemh_status aemh_autotune(aemh_ctx* ctx);

@@ -1,0 +1,322 @@
+"""All-integer MLS-MPM solver on FixedPoint Q32.32 (ae_physics/src/mpm.rs
+MpmSimulation).
+
+CPU: the oracle (oracle/fixedsim.cpp) against the reference's own unit tests
+of FixedPoint (fixed_point.rs:889-962) and MpmSimulation (mpm.rs:957-1148),
+and against exact closed forms for what those leave open (Python integers
+restate mul / div / sqrt / the table; the first P2G deposit of a resting
+particle; the free-fall velocity after one substep).
+GPU: aemh_fixed_* through the C ABI against the oracle, every field of every
+particle bit-exact, for each material model, substeps, added and removed
+particles and fracture.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import fixedsim as X
+
+ONE = 1 << 32
+I64MAX, I64MIN = (1 << 63) - 1, -(1 << 63)
+
+
+def trunc_div(a, b):
+    q = abs(a) // abs(b)
+    return q if (a >= 0) == (b >= 0) else -q
+
+
+def wrap64(v):
+    v &= (1 << 64) - 1
+    return v - (1 << 64) if v >= 1 << 63 else v
+
+
+# ---------------------------------------------------------------------------
+# FixedPoint: the reference's unit tests (fixed_point.rs:889-962)
+# ---------------------------------------------------------------------------
+def f32(r):
+    return X.to_f32(r)
+
+
+def test_fixed_trig():
+    pi = 13493037704
+    half_pi = X.op("div", pi, 2 * ONE)
+    assert abs(f32(X.op("sin", 0))) < 1e-3
+    assert abs(f32(X.op("sin", half_pi)) - 1.0) < 1e-3
+    assert abs(f32(X.op("cos", 0)) - 1.0) < 1e-3
+    assert abs(f32(X.op("cos", half_pi))) < 1e-3
+
+
+def test_fixed_sqrt_and_arithmetic():
+    assert abs(f32(X.op("sqrt", 4 * ONE)) - 2.0) < 1e-3
+    assert abs(f32(X.op("sqrt", 2 * ONE)) - 1.414) < 1e-2
+    a, b = X.from_f32(3.5), X.from_f32(2.0)
+    assert abs(f32(X.op("add", a, b)) - 5.5) < 1e-3
+    assert abs(f32(X.op("sub", a, b)) - 1.5) < 1e-3
+    assert abs(f32(X.op("mul", a, b)) - 7.0) < 1e-3
+    assert abs(f32(X.op("div", a, b)) - 1.75) < 1e-3
+    third = X.from_f32(1.0 / 3.0)
+    assert abs(f32(X.op("mul", third, 3 * ONE)) - 1.0) < 1e-3  # determinism test
+    assert X.op("add", I64MAX, ONE) == I64MAX  # overflow saturates (Add, :187-189)
+
+
+# ---------------------------------------------------------------------------
+# FixedPoint: exact forms
+# ---------------------------------------------------------------------------
+def test_fixed_mul_div_sqrt_exact():
+    rng = np.random.default_rng(1)
+    vals = [int(v) for v in rng.integers(-(1 << 50), 1 << 50, 200)] + [0, 1, -1, ONE, -ONE, I64MAX,
+                                                                         I64MIN]
+    for a in vals:
+        for b in vals[:40]:
+            assert X.op("mul", a, b) == wrap64(trunc_div(a * b, 1 << 32))
+            want = (0 if a == 0 else (I64MAX if a > 0 else I64MIN)) if b == 0 else \
+                wrap64(trunc_div(a * (1 << 32), b))
+            assert X.op("div", a, b) == want
+        want = 0 if a <= 0 else math.isqrt(a << 32)
+        assert X.op("sqrt", a) == wrap64(want)
+        assert X.op("neg", a) == (I64MAX if a == I64MIN else -a)
+
+
+def test_fixed_sin_table_and_interpolation():
+    # table entries: (sin(i/1024 pi/2) 2^32) truncated; within 1 raw of the
+    # correctly rounded value computed with exact Python arithmetic
+    for i in (0, 1, 100, 511, 512, 1000, 1023):
+        want = math.sin(i / 1024 * math.pi / 2) * 2 ** 32
+        assert abs(X.lib().fx_sin_table(i) - want) <= 1.0
+    # between two table nodes sin interpolates linearly (fixed_point.rs:126-128)
+    step = 1 << 22
+    for i in (3, 200, 700):
+        lo, hi = X.lib().fx_sin_table(i), X.lib().fx_sin_table(i + 1)
+        ang = i * step + step // 4
+        assert X.op("sin", ang) == lo + trunc_div((hi - lo) * (step // 4), step)
+        assert X.op("sin", -ang) == -X.op("sin", ang)  # odd
+
+
+def test_fixed_from_to_f32():
+    for v in (0.0, 1.5, -2.25, 1e-9, 3.0e8, -7.0e8):
+        assert X.from_f32(v) == int(np.float64(np.float32(v)) * 2 ** 32)
+    assert X.from_f32(float("nan")) == 0
+    assert X.from_f32(3.0e9) == I64MAX and X.from_f32(-1e30) == I64MIN
+    assert X.to_f32(3 * ONE + ONE // 2) == 3.5
+
+
+def test_fixed_ln_exp_against_math():
+    # ln is 2 artanh((x-1)/(x+1)) cut after the t^9 term (fixed_point.rs:254-271):
+    # the cut series in float, and math.log near 1 where the cut is negligible
+    for v in (0.1, 0.5, 1.0, 2.0, 5.0, 40.0):
+        t = (v - 1.0) / (v + 1.0)
+        cut = 2.0 * sum(t ** k / k for k in (1, 3, 5, 7, 9))
+        assert abs(f32(X.op("ln", X.from_f32(v))) - cut) < 1e-6 * max(1.0, abs(cut))
+    for v in (0.9, 1.1, 1.5):
+        assert abs(f32(X.op("ln", X.from_f32(v))) - math.log(v)) < 1e-5
+    for v in (-1.0, -0.1, 0.3, 1.0):
+        assert abs(f32(X.op("exp", X.from_f32(v))) - math.exp(v)) < 1e-5 * math.exp(v) + 1e-6
+    assert X.op("ln", 0) == I64MIN and X.op("exp", 0) == ONE
+
+
+def test_fixed_det_inverse_svd():
+    m = np.array([2, 0, 0, 0, 3, 0, 0, 0, 4], np.int64) * ONE  # columns
+    assert X.mat_op(0, m)[0] == 24 * ONE
+    inv = X.mat_op(1, m)[:9]
+    # adjugate entry times ONE / det, each product truncated (fixed_point.rs:730-762)
+    inv_det = (ONE << 32) // (24 * ONE)
+    assert list(inv) == [12 * inv_det, 0, 0, 0, 8 * inv_det, 0, 0, 0, 6 * inv_det]
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        a = rng.uniform(-2, 2, (3, 3)) + 3 * np.eye(3)
+        raw = np.array([X.from_f32(v) for v in a.T.reshape(9)], np.int64)  # columns
+        det = X.mat_op(0, raw)[0] / 2 ** 32
+        assert abs(det - np.linalg.det(a.astype(np.float32).astype(np.float64))) < 1e-6 * 64
+        inv = X.mat_op(1, raw)[:9].reshape(3, 3).T / 2 ** 32
+        assert np.abs(inv - np.linalg.inv(a.astype(np.float32).astype(np.float64))).max() < 1e-6
+    u, s, v = X.mat_op(2, m).reshape(3, 9)
+    # diagonal input: no rotation, singular values are the diagonal, and V is
+    # the reference's rotated A^T A (diag of squares), not its rotations
+    assert list(s[[0, 4, 8]]) == [2 * ONE, 3 * ONE, 4 * ONE]
+    assert list(v[[0, 4, 8]]) == [4 * ONE, 9 * ONE, 16 * ONE]
+    big = np.full(9, I64MAX, np.int64)
+    assert X.mat_op(0, big)[0] == 0  # overflowing i128 pieces become 0 (:711-729)
+
+
+# ---------------------------------------------------------------------------
+# MpmSimulation: the reference's unit tests (mpm.rs:957-1148)
+# ---------------------------------------------------------------------------
+def test_sim_initialization_and_snapshots():
+    s = X.Sim(X.small_config())
+    d = s.read()
+    assert s.count() == 8 and s.active_count() == 8
+    assert (d["q"][:, 24] > 0).all() and (d["q"][:, 25] > 0).all()  # mass, volume
+
+
+def test_sim_substeps_and_models_run():
+    s = X.Sim(X.small_config())
+    s.step(10)
+    assert s.state()["step_count"] == 10 and s.active_count() == 8
+    for model, param in ((X.FLUID, X.from_f32(0.01)), (X.GRANULAR, X.from_f32(30.0)),
+                         (X.SNOW, X.from_f32(0.05)), (X.BRITTLE, X.from_f32(0.02))):
+        t = X.Sim(X.small_config(model=model, model_param=param))
+        t.step(5)
+        assert t.active_count() > 0
+
+
+def test_sim_elastic_kinetic_energy_rises():
+    s = X.Sim(X.small_config(model=X.ELASTIC))
+    e0 = s.diag()["kinetic_energy"]
+    s.step(5)
+    assert s.active_count() > 0 and s.diag()["kinetic_energy"] > e0
+
+
+def test_sim_mass_conservation_and_fracture_enabled():
+    s = X.Sim(X.small_config())
+    m0 = s.diag()["total_mass"]
+    s.step(5)
+    assert abs(m0 - s.diag()["total_mass"]) < X.op("mul", m0, X.from_f32(0.001))
+    f = X.Sim(X.small_config(enable_fracture=1, fracture_strain=X.from_f32(0.01)))
+    f.step(5)
+    assert f.active_count() > 0
+
+
+def test_sim_add_remove_and_switch_models():
+    s = X.Sim(X.small_config())
+    pid = s.add_particle((X.from_f32(0.5),) * 3, (0, 0, 0), X.from_f32(0.1))
+    assert s.active_count() == 9
+    assert s.remove_particle(pid) and s.active_count() == 8
+    assert not s.remove_particle(99999)
+    for m in (X.ELASTIC, X.ELASTOPLASTIC, X.FLUID):
+        s.set_model(m, X.from_f32(0.01))
+        s.step()
+    assert s.state()["step_count"] == 3
+    z = X.Sim(X.default_config(particle_count=0, res=(4, 4, 4)))
+    z.step()
+    assert z.count() == 0
+
+
+# ---------------------------------------------------------------------------
+# MpmSimulation: closed forms
+# ---------------------------------------------------------------------------
+def test_sim_free_fall_one_substep_closed_form():
+    """Elastic model, one particle at rest at a cell corner with F = I: stress
+    is zero, P2G deposits m w per node, the grid force adds g dt m per node,
+    G2P gives v = sum_w (g dt m w)/(m w)... computed here with Python
+    integers from the same rules, so a dropped term or a wrong order fails."""
+    cfg = X.small_config(model=X.ELASTIC, particle_count=0)
+    s = X.Sim(cfg)
+    pos = (X.from_f32(1.0), X.from_f32(1.25), X.from_f32(1.0))
+    mass = X.from_f32(0.5)
+    s.add_particle(pos, (0, 0, 0), mass)
+    s.step(1)
+    q = s.read()["q"][0]
+    # expected, with exact integer restatements of the FixedPoint rules
+    mul = lambda a, b: wrap64(trunc_div(a * b, 1 << 32))  # noqa: E731
+    div = lambda a, b: wrap64(trunc_div(a << 32 if a >= 0 else -((-a) << 32), b))  # noqa: E731
+    cell, dt, g = cfg.cell, cfg.dt, cfg.gravity[1]
+    inv = div(ONE, cell)
+
+    def bsp(f):
+        a = abs(f)
+        if a < ONE // 2:
+            return (3 * ONE // 4) - mul(f, f)
+        if a < 3 * ONE // 2:
+            x = 3 * ONE // 2 - a
+            return mul(mul(ONE // 2, x), x)
+        return 0
+    fx = [mul(p, inv) - (mul(p, inv) & ~0xFFFFFFFF) for p in pos]
+    base = [(mul(p, inv) & ~0xFFFFFFFF) >> 32 for p in pos]
+    vy = 0
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                gx, gy, gz = base[0] + dx, base[1] + dy, base[2] + dz
+                if not all(0 <= c < 4 for c in (gx, gy, gz)):
+                    continue
+                w = mul(mul(bsp(fx[0] - dx * ONE), bsp(fx[1] - dy * ONE)), bsp(fx[2] - dz * ONE))
+                if w < 10:
+                    continue
+                gm = mul(mass, w)
+                if gm < 10:
+                    continue
+                gvel = div(mul(mul(g, dt), gm), gm)  # momentum 0 + g dt m, then / m
+                vy += mul(gvel, w)
+    assert q[4] == vy and q[3] == 0 and q[5] == 0
+    assert q[1] == pos[1] + mul(vy, dt)
+
+
+# ---------------------------------------------------------------------------
+# GPU against the oracle
+# ---------------------------------------------------------------------------
+def _gpu_pair(cfg):
+    import aemh
+    return aemh.FixedSim(cfg), X.Sim(cfg)
+
+
+def _compare(g, o, where):
+    dg, do = g.read(), o.read()
+    assert dg["q"].shape == do["q"].shape, where
+    bad = np.argwhere(dg["q"] != do["q"])
+    assert bad.size == 0, f"{where}: {len(bad)} values differ, first {bad[:3].tolist()}"
+    assert np.array_equal(dg["flags"], do["flags"]), where
+    assert np.array_equal(dg["ids"], do["ids"]), where
+    assert np.array_equal(dg["color"], do["color"]), where
+    sg, so = g.state(), o.state()
+    for k in ("step_count", "next_id", "n_events", "hardening", "fracture_strain", "fracture_on"):
+        assert sg[k] == so[k], (where, k, sg[k], so[k])
+    assert sg["time"] == so["time"], where
+
+
+MODELS = [(X.ELASTIC, 0), (X.ELASTOPLASTIC, 0), (X.GRANULAR, X.from_f32(30.0)),
+          (X.FLUID, X.from_f32(0.01)), (X.SNOW, X.from_f32(0.05)), (X.BRITTLE, X.from_f32(0.02))]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model,param", MODELS, ids=["elastic", "elastoplastic", "granular",
+                                                     "fluid", "snow", "brittle"])
+def test_gpu_fixed_models_bitexact(model, param):
+    cfg = X.small_config(model=model, model_param=param, particle_count=64, res=(8, 8, 8),
+                         cell=X.from_f32(0.25), substeps=2)
+    g, o = _gpu_pair(cfg)
+    for k in range(4):
+        g.step(3)
+        o.step(3)
+        _compare(g, o, f"model {model} step {3 * (k + 1)}")
+    dg, do = g.diag(), o.diag()
+    assert dg == do
+    g.close()
+
+
+@pytest.mark.gpu
+def test_gpu_fixed_reference_config_add_remove_fracture():
+    cfg = X.small_config(enable_fracture=1, fracture_strain=X.from_f32(0.01))
+    g, o = _gpu_pair(cfg)
+    g.step(2)
+    o.step(2)
+    _compare(g, o, "fracture 2")
+    for s in (g, o):
+        s.add_particle((X.from_f32(0.7), X.from_f32(1.1), X.from_f32(0.6)), (0, X.from_f32(-0.5), 0),
+                       X.from_f32(0.05))
+    g.step(3)
+    o.step(3)
+    _compare(g, o, "fracture 5")
+    ev_g, ev_o = g.events(), o.events()
+    for k in ev_o:
+        assert np.array_equal(ev_g[k], ev_o[k]), k
+    pid = int(o.read()["ids"][2, 0])
+    assert g.remove_particle(pid) == o.remove_particle(pid) is True
+    g.set_model(X.SNOW, X.from_f32(0.05))
+    o.set_model(X.SNOW, X.from_f32(0.05))
+    g.step(2)
+    o.step(2)
+    _compare(g, o, "after remove + snow")
+    g.close()
+
+
+@pytest.mark.gpu
+def test_gpu_fixed_default_config_many_particles():
+    """MpmConfig::default: 10,000 particles (22^3 = 10,648 placed), 64^3 grid,
+    4 substeps per step."""
+    cfg = X.default_config()
+    g, o = _gpu_pair(cfg)
+    g.step(2)
+    o.step(2)
+    _compare(g, o, "default config")
+    g.close()
