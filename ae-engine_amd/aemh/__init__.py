@@ -96,6 +96,49 @@ class Scene(C.Structure):
     ]
 
 
+# --- all-integer MpmSimulation (ae_physics/src/mpm.rs) ----------------------
+FX_ELASTIC, FX_ELASTOPLASTIC, FX_GRANULAR, FX_BRITTLE, FX_FLUID, FX_SNOW = range(6)
+Q32_ONE = 1 << 32
+
+
+class FixedConfig(C.Structure):
+    """MpmConfig (ae_physics/src/mpm.rs:15-31), FixedPoint fields as raw int64."""
+    _fields_ = [("grid_resolution", C.c_uint32 * 3), ("cell_size", C.c_int64),
+                ("youngs_modulus", C.c_int64), ("poissons_ratio", C.c_int64),
+                ("yield_stress", C.c_int64), ("hardening", C.c_int64), ("density", C.c_int64),
+                ("gravity", C.c_int64 * 3), ("dt", C.c_int64), ("substeps", C.c_uint32),
+                ("material_model", C.c_uint32), ("model_param", C.c_int64),
+                ("enable_fracture", C.c_uint32), ("fracture_strain", C.c_int64),
+                ("particle_count", C.c_uint64)]
+
+
+class FixedState(C.Structure):
+    _fields_ = [("particles", C.c_uint64), ("step_count", C.c_uint64), ("time", C.c_double),
+                ("next_particle_id", C.c_uint64), ("fracture_events", C.c_uint64),
+                ("hardening", C.c_int64), ("fracture_strain", C.c_int64),
+                ("enable_fracture", C.c_uint32), ("material_model", C.c_uint32),
+                ("model_param", C.c_int64), ("checked_substeps", C.c_uint64),
+                ("serial_substeps", C.c_uint64), ("max_scatter_bound", C.c_uint64)]
+
+
+class FractureEvent(C.Structure):
+    _fields_ = [("time", C.c_double), ("position", C.c_float * 3),
+                ("strain_magnitude", C.c_float), ("num_fragments", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class FixedStats(C.Structure):
+    _fields_ = [("active_particles", C.c_uint64), ("fractured_count", C.c_uint64),
+                ("total_mass", C.c_int64), ("average_temperature", C.c_int64),
+                ("kinetic_energy", C.c_int64), ("total_strain_energy", C.c_int64)]
+
+
+class FixedSnapshot(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("velocity", C.c_float * 3),
+                ("stress_magnitude", C.c_float), ("plastic_strain", C.c_float),
+                ("temperature", C.c_float), ("color", C.c_float * 4), ("fractured", C.c_uint32)]
+
+
 # Every symbol include/aemh.h declares; tests check the library exports them.
 EXPORTS = [
     "aemh_config_default", "aemh_version", "aemh_create", "aemh_destroy", "aemh_last_error",
@@ -111,6 +154,12 @@ EXPORTS = [
     "aemh_mpss_update_lifetimes", "aemh_mpss_write", "aemh_mpss_read", "aemh_mpss_state",
     "aemh_lod_config_default", "aemh_tier_set_config", "aemh_tier_tick", "aemh_tier_lists",
     "aemh_tier_render",
+    "aemh_fixed_config_default", "aemh_fixed_create", "aemh_fixed_destroy",
+    "aemh_fixed_last_error", "aemh_fixed_add_particle", "aemh_fixed_add_particles",
+    "aemh_fixed_remove_particle", "aemh_fixed_step", "aemh_fixed_set_material_model",
+    "aemh_fixed_enable_fracture", "aemh_fixed_disable_fracture", "aemh_fixed_get_state",
+    "aemh_fixed_read_particles", "aemh_fixed_write_particles", "aemh_fixed_fracture_events",
+    "aemh_fixed_get_stats", "aemh_fixed_snapshots",
 ]
 
 NO_SLOT = 0xFFFFFFFF  # spawn() on a full buffer (None in the reference)
@@ -178,10 +227,35 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.aemh_tier_tick.argtypes = [P, fp, C.c_float, C.POINTER(TierStats)]
     lib.aemh_tier_lists.argtypes = [P, pu32, pu32, pu32, pu32]
     lib.aemh_tier_render.argtypes = [P, u32, u32, fp, pu32, pu32]
+    i64p = C.POINTER(C.c_int64)
+    u64p = C.POINTER(C.c_uint64)
+    lib.aemh_fixed_config_default.argtypes = [C.POINTER(FixedConfig)]
+    lib.aemh_fixed_create.argtypes = [C.POINTER(P), C.c_int, C.POINTER(FixedConfig)]
+    lib.aemh_fixed_destroy.argtypes = [P]
+    lib.aemh_fixed_last_error.restype = C.c_char_p
+    lib.aemh_fixed_last_error.argtypes = [P]
+    lib.aemh_fixed_add_particle.argtypes = [P, i64p, i64p, C.c_int64, fp, u64p]
+    lib.aemh_fixed_add_particles.argtypes = [P, C.c_uint64, i64p, i64p, i64p, fp, u64p]
+    lib.aemh_fixed_remove_particle.argtypes = [P, C.c_uint64, C.POINTER(C.c_int)]
+    lib.aemh_fixed_step.argtypes = [P, u32]
+    lib.aemh_fixed_set_material_model.argtypes = [P, u32, C.c_int64]
+    lib.aemh_fixed_enable_fracture.argtypes = [P, C.c_int64]
+    lib.aemh_fixed_disable_fracture.argtypes = [P]
+    lib.aemh_fixed_get_state.argtypes = [P, C.POINTER(FixedState)]
+    lib.aemh_fixed_read_particles.argtypes = [P, C.c_uint64, C.c_uint64, i64p,
+                                              C.POINTER(C.c_uint8), u64p, fp]
+    lib.aemh_fixed_write_particles.argtypes = [P, C.c_uint64, C.c_uint64, i64p,
+                                               C.POINTER(C.c_uint8), u64p, fp]
+    lib.aemh_fixed_fracture_events.argtypes = [P, C.c_uint64, C.c_uint64,
+                                               C.POINTER(FractureEvent)]
+    lib.aemh_fixed_get_stats.argtypes = [P, C.POINTER(FixedStats)]
+    lib.aemh_fixed_snapshots.argtypes = [P, C.POINTER(FixedSnapshot), C.c_uint64, u64p]
     for name in EXPORTS:
         f = getattr(lib, name)
         if name not in ("aemh_version", "aemh_last_error", "aemh_config_default",
-                        "aemh_scene_uniform_box", "aemh_lod_config_default"):
+                        "aemh_scene_uniform_box", "aemh_lod_config_default",
+                        "aemh_fixed_config_default", "aemh_fixed_destroy",
+                        "aemh_fixed_last_error"):
             f.restype = C.c_int
     _lib = lib
     return lib
@@ -555,3 +629,141 @@ class MpssBuffer:
 
     def read(self, slots, fields=tuple(MPSS_FIELDS)):
         return self.engine.mpss_read(slots, fields)
+
+
+def fixed_config_default() -> FixedConfig:
+    """MpmConfig::default (ae_physics/src/mpm.rs:33-52)."""
+    cfg = FixedConfig()
+    load_library().aemh_fixed_config_default(C.byref(cfg))
+    return cfg
+
+
+class FixedSim:
+    """MpmSimulation (ae_physics/src/mpm.rs:116-792) resident on one gfx950
+    device; FixedPoint values are raw int64 (value = raw / 2^32).
+
+    `cfg` is a FixedConfig or any ctypes structure with its layout."""
+
+    def __init__(self, cfg, device: int = 0):
+        self._lib = load_library()
+        if not isinstance(cfg, FixedConfig):
+            cfg = FixedConfig.from_buffer_copy(bytes(cfg))
+        h = C.c_void_p()
+        st = self._lib.aemh_fixed_create(C.byref(h), device, C.byref(cfg))
+        if st != OK:
+            raise AemhError(st, "aemh_fixed_create failed (bad config, no gfx950 device or HIP "
+                                "unusable)")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.aemh_fixed_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != OK:
+            raise AemhError(st, self._lib.aemh_fixed_last_error(self._h).decode())
+
+    def step(self, n: int = 1):
+        self._check(self._lib.aemh_fixed_step(self._h, n))
+
+    def state(self) -> dict:
+        s = FixedState()
+        self._check(self._lib.aemh_fixed_get_state(self._h, C.byref(s)))
+        return dict(count=s.particles, step_count=s.step_count, time=s.time,
+                    next_id=s.next_particle_id, n_events=s.fracture_events,
+                    hardening=s.hardening, fracture_strain=s.fracture_strain,
+                    fracture_on=s.enable_fracture, model=s.material_model,
+                    model_param=s.model_param, checked_substeps=s.checked_substeps,
+                    serial_substeps=s.serial_substeps, max_scatter_bound=s.max_scatter_bound)
+
+    def count(self) -> int:
+        return self.state()["count"]
+
+    def add_particle(self, pos, vel, mass, color=(1.0, 0.0, 0.0, 1.0)) -> int:
+        p = (C.c_int64 * 3)(*[int(v) for v in pos])
+        v = (C.c_int64 * 3)(*[int(x) for x in vel])
+        col = (C.c_float * 4)(*color)
+        pid = C.c_uint64()
+        self._check(self._lib.aemh_fixed_add_particle(self._h, p, v, int(mass), col,
+                                                       C.byref(pid)))
+        return pid.value
+
+    def add_particles(self, pos, vel, mass, color=None) -> int:
+        pos = np.ascontiguousarray(pos, np.int64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, np.int64).reshape(-1, 3)
+        mass = np.ascontiguousarray(mass, np.int64).reshape(-1)
+        col = None if color is None else np.ascontiguousarray(color, np.float32).reshape(-1, 4)
+        first = C.c_uint64()
+        i64p = C.POINTER(C.c_int64)
+        self._check(self._lib.aemh_fixed_add_particles(
+            self._h, len(mass), pos.ctypes.data_as(i64p), vel.ctypes.data_as(i64p),
+            mass.ctypes.data_as(i64p), _f(col), C.byref(first)))
+        return first.value
+
+    def remove_particle(self, pid: int) -> bool:
+        r = C.c_int()
+        self._check(self._lib.aemh_fixed_remove_particle(self._h, C.c_uint64(pid), C.byref(r)))
+        return bool(r.value)
+
+    def set_model(self, model: int, param: int = 0):
+        self._check(self._lib.aemh_fixed_set_material_model(self._h, model, int(param)))
+
+    def set_fracture(self, on: bool, threshold: int = 0):
+        if on:
+            self._check(self._lib.aemh_fixed_enable_fracture(self._h, int(threshold)))
+        else:
+            self._check(self._lib.aemh_fixed_disable_fracture(self._h))
+
+    def read(self, first: int = 0, count=None) -> dict:
+        n = self.count()
+        count = n - first if count is None else count
+        q = np.zeros((max(count, 1), 29), np.int64)
+        fl = np.zeros((max(count, 1), 3), np.uint8)
+        ids = np.zeros((max(count, 1), 9), np.uint64)
+        col = np.zeros((max(count, 1), 4), np.float32)
+        self._check(self._lib.aemh_fixed_read_particles(
+            self._h, first, count, q.ctypes.data_as(C.POINTER(C.c_int64)),
+            fl.ctypes.data_as(C.POINTER(C.c_uint8)), ids.ctypes.data_as(C.POINTER(C.c_uint64)),
+            _f(col)))
+        return dict(q=q[:count], flags=fl[:count], ids=ids[:count], color=col[:count])
+
+    def write(self, first: int, q, flags, ids, color):
+        q = np.ascontiguousarray(q, np.int64).reshape(-1, 29)
+        fl = np.ascontiguousarray(flags, np.uint8).reshape(-1, 3)
+        ids = np.ascontiguousarray(ids, np.uint64).reshape(-1, 9)
+        col = np.ascontiguousarray(color, np.float32).reshape(-1, 4)
+        self._check(self._lib.aemh_fixed_write_particles(
+            self._h, first, len(q), q.ctypes.data_as(C.POINTER(C.c_int64)),
+            fl.ctypes.data_as(C.POINTER(C.c_uint8)), ids.ctypes.data_as(C.POINTER(C.c_uint64)),
+            _f(col)))
+
+    def events(self) -> dict:
+        ne = self.state()["n_events"]
+        ev = (FractureEvent * max(ne, 1))()
+        self._check(self._lib.aemh_fixed_fracture_events(self._h, 0, ne, ev))
+        return dict(time=np.array([e.time for e in ev[:ne]], np.float64),
+                    pos=np.array([list(e.position) for e in ev[:ne]], np.float32).reshape(-1, 3),
+                    strain=np.array([e.strain_magnitude for e in ev[:ne]], np.float32),
+                    frags=np.array([e.num_fragments for e in ev[:ne]], np.uint32))
+
+    def stats(self) -> dict:
+        s = FixedStats()
+        self._check(self._lib.aemh_fixed_get_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in FixedStats._fields_}
+
+    def snapshots(self) -> list:
+        n = C.c_uint64()
+        self._check(self._lib.aemh_fixed_snapshots(self._h, None, 0, C.byref(n)))
+        out = (FixedSnapshot * max(n.value, 1))()
+        self._check(self._lib.aemh_fixed_snapshots(self._h, out, n.value, C.byref(n)))
+        return list(out[:n.value])
+
+    def active_count(self) -> int:
+        return self.stats()["active_particles"]

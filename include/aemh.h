@@ -298,6 +298,150 @@ aemh_status aemh_profile_reset(aemh_ctx* ctx);
 void aemh_scene_uniform_box(uint64_t seed, uint32_t n, const float lo[3],
                             const float hi[3], float* pos3);
 
+/* ------------------------------------------------------------------------
+ * All-integer MLS-MPM solver: ae_physics::mpm::MpmSimulation
+ * (ae_physics/src/mpm.rs:116-792) on FixedPoint Q32.32
+ * (ae_physics/src/fixed_point.rs).  Every FixedPoint crosses the boundary
+ * as its raw int64 (value = raw / 2^32).  Results are bit-identical to the
+ * reference's serial loops: particle order, ids, fracture appends and
+ * events included.  The grid scatter adds integers in parallel, which is
+ * exact while no saturating prefix sum of the reference's serial order can
+ * clip: the device proves that from bounds on the sums of |contribution|
+ * (whole substep, then per cell) and otherwise rebuilds that substep's grid
+ * with the serial loop (counted in aemh_fixed_state).
+ * ---------------------------------------------------------------------- */
+typedef struct aemh_fixed aemh_fixed;
+
+/* MpmMaterialModel (mpm.rs:5-13); model_param carries the variant's field */
+typedef enum {
+  AEMH_FX_ELASTIC = 0,
+  AEMH_FX_ELASTOPLASTIC = 1,
+  AEMH_FX_GRANULAR = 2,  /* model_param: friction_angle (degrees)   */
+  AEMH_FX_BRITTLE = 3,   /* model_param: fracture_strain           */
+  AEMH_FX_FLUID = 4,     /* model_param: viscosity                 */
+  AEMH_FX_SNOW = 5       /* model_param: hardening                 */
+} aemh_fixed_model;
+
+/* MpmConfig (mpm.rs:15-31) */
+typedef struct {
+  uint32_t grid_resolution[3];
+  int64_t cell_size;
+  int64_t youngs_modulus;
+  int64_t poissons_ratio;
+  int64_t yield_stress;
+  int64_t hardening;
+  int64_t density;
+  int64_t gravity[3];
+  int64_t dt;
+  uint32_t substeps;
+  uint32_t material_model; /* aemh_fixed_model */
+  int64_t model_param;
+  uint32_t enable_fracture;
+  int64_t fracture_strain;
+  uint64_t particle_count;
+} aemh_fixed_config;
+
+/* The non-particle fields of MpmSimulation (mpm.rs:116-126) and the config
+ * fields the material models change while stepping (:464-470). */
+typedef struct {
+  uint64_t particles;            /* particles.len()                        */
+  uint64_t step_count;
+  double time;
+  uint64_t next_particle_id;
+  uint64_t fracture_events;      /* fracture_events.len()                  */
+  int64_t hardening;
+  int64_t fracture_strain;
+  uint32_t enable_fracture;
+  uint32_t material_model;
+  int64_t model_param;
+  uint64_t checked_substeps;     /* substeps whose scatter was bounded per cell */
+  uint64_t serial_substeps;      /* substeps whose grid was rebuilt serially    */
+  /* largest whole-substep sum of ((|c| >> 24) + 1) over nonzero scatter
+   * contributions c; the atomic scatter alone is proven exact below 2^39
+   * (UINT64_MAX once a substep went past it) */
+  uint64_t max_scatter_bound;
+} aemh_fixed_state;
+
+/* FractureEvent (mpm.rs:132-138) */
+typedef struct {
+  double time;
+  float position[3];
+  float strain_magnitude;
+  uint32_t num_fragments;
+  uint32_t pad;
+} aemh_fracture_event;
+
+/* active_particles, fractured_count, total_mass, average_temperature,
+ * kinetic_energy, total_strain_energy (mpm.rs:744-778) */
+typedef struct {
+  uint64_t active_particles;
+  uint64_t fractured_count;
+  int64_t total_mass;
+  int64_t average_temperature;
+  int64_t kinetic_energy;
+  int64_t total_strain_energy;
+} aemh_fixed_stats;
+
+/* MpmParticleSnapshot (mpm.rs:94-103) */
+typedef struct {
+  float position[3];
+  float velocity[3];
+  float stress_magnitude;
+  float plastic_strain;
+  float temperature;
+  float color[4];
+  uint32_t fractured;
+} aemh_fixed_snapshot;
+
+/* MpmConfig::default (mpm.rs:33-52). */
+void aemh_fixed_config_default(aemh_fixed_config* cfg);
+/* MpmSimulation::new (mpm.rs:159-177): the grid, and config.particle_count
+ * particles placed by init_particles (:179-206). */
+aemh_status aemh_fixed_create(aemh_fixed** out, int device, const aemh_fixed_config* cfg);
+void aemh_fixed_destroy(aemh_fixed* sim);
+const char* aemh_fixed_last_error(const aemh_fixed* sim);
+/* add_particle (mpm.rs:212-233): pos3, vel3 raw, mass raw, color4; the id
+ * is written to *id_out (may be NULL). */
+aemh_status aemh_fixed_add_particle(aemh_fixed* sim, const int64_t* pos3, const int64_t* vel3,
+                                    int64_t mass, const float* color4, uint64_t* id_out);
+/* n add_particle calls in order, in one upload: pos3/vel3 [n][3], mass [n],
+ * color4 [n][4] (NULL: the default color).  *first_id_out gets the first
+ * id (ids are consecutive).  No reference counterpart beyond the loop. */
+aemh_status aemh_fixed_add_particles(aemh_fixed* sim, uint64_t n, const int64_t* pos3,
+                                     const int64_t* vel3, const int64_t* mass,
+                                     const float* color4, uint64_t* first_id_out);
+/* remove_particle (mpm.rs:235-242): *removed = 1 if the id was found. */
+aemh_status aemh_fixed_remove_particle(aemh_fixed* sim, uint64_t id, int* removed);
+/* step (mpm.rs:244-251), n_steps times. */
+aemh_status aemh_fixed_step(aemh_fixed* sim, uint32_t n_steps);
+/* set_material_model (mpm.rs:780-782). */
+aemh_status aemh_fixed_set_material_model(aemh_fixed* sim, uint32_t model, int64_t model_param);
+/* enable_fracture / disable_fracture (mpm.rs:784-791). */
+aemh_status aemh_fixed_enable_fracture(aemh_fixed* sim, int64_t threshold);
+aemh_status aemh_fixed_disable_fracture(aemh_fixed* sim);
+aemh_status aemh_fixed_get_state(aemh_fixed* sim, aemh_fixed_state* out);
+/* The public `particles` vector (mpm.rs:119), records [first, first+count):
+ * q29 [count][29] raw int64 = position3, velocity3, deformation_gradient9
+ * (column-major: x_axis, y_axis, z_axis), stress9 (column-major), mass,
+ * volume, plastic_strain, temperature, accumulated_strain; flags3 [count][3]
+ * = active, fractured, num_connections; ids9 [count][9] = parent_id,
+ * connected_to[8]; color4 [count][4].  NULL outputs are skipped. */
+aemh_status aemh_fixed_read_particles(aemh_fixed* sim, uint64_t first, uint64_t count, int64_t* q29,
+                                      uint8_t* flags3, uint64_t* ids9, float* color4);
+/* Overwrites records [first, first+count) (first+count <= particles), as a
+ * caller assigning the public fields does.  All four arrays required. */
+aemh_status aemh_fixed_write_particles(aemh_fixed* sim, uint64_t first, uint64_t count,
+                                       const int64_t* q29, const uint8_t* flags3,
+                                       const uint64_t* ids9, const float* color4);
+/* fracture_events (mpm.rs:125) [first, first+count). */
+aemh_status aemh_fixed_fracture_events(aemh_fixed* sim, uint64_t first, uint64_t count,
+                                       aemh_fracture_event* out);
+aemh_status aemh_fixed_get_stats(aemh_fixed* sim, aemh_fixed_stats* out);
+/* snapshots (mpm.rs:718-742): active particles in order, at most cap;
+ * *n_out = the number of active particles. */
+aemh_status aemh_fixed_snapshots(aemh_fixed* sim, aemh_fixed_snapshot* out, uint64_t cap,
+                                 uint64_t* n_out);
+
 #ifdef __cplusplus
 }
 #endif

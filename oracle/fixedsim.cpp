@@ -393,7 +393,11 @@ struct Sim {
     if (f <= -2147483648.0f) return INT32_MIN;
     return (int32_t)f;
   }
+  // sum over the substep's nonzero scatter contributions c of
+  // (|c| >> 24) + 1, the quantity the device bounds; max over substeps
+  uint64_t sum_bound = 0, max_bound = 0;
   void p2g() {
+    sum_bound = 0;
     const Fx inv = div(ONE, cell);
     for (const Particle& q : p) {
       if (!q.active) continue;
@@ -414,13 +418,18 @@ struct Sim {
             V3 rel = vsub(gp, q.pos);
             V3 str = mvmul(q.stress, rel);
             V3 mom = vmul(vadd(q.vel, vmul(vmul(str, inv), inv)), q.mass);
-            grid[idx].mass = add(grid[idx].mass, mul(q.mass, w));
-            grid[idx].vel = vadd(grid[idx].vel, vmul(mom, w));
+            const Fx dm = mul(q.mass, w);
+            const V3 dv = vmul(mom, w);
+            grid[idx].mass = add(grid[idx].mass, dm);
+            grid[idx].vel = vadd(grid[idx].vel, dv);
             grid[idx].active = true;
+            for (int64_t c : {dm.raw, dv.x.raw, dv.y.raw, dv.z.raw})
+              if (c != 0) sum_bound += ((c < 0 ? 0 - (uint64_t)c : (uint64_t)c) >> 24) + 1;
           }
     }
   }
   void grid_forces(Fx sdt) {
+    if (sum_bound > max_bound) max_bound = sum_bound;
     for (Cell& c : grid) {
       if (!c.active || c.mass.raw < FP_EPS.raw) continue;
       V3 force = vmul(vmul(gravity, sdt), c.mass);
@@ -752,6 +761,28 @@ void fx_sim_read(void* h, int64_t* q29, uint8_t* flags3, uint64_t* ids9, float* 
     memcpy(color4 + i * 4, q.color, 16);
   }
 }
+// assigns records [first, first+count) of the public `particles` vector
+void fx_sim_write(void* h, uint64_t first, uint64_t count, const int64_t* q29, const uint8_t* flags3,
+                  const uint64_t* ids9, const float* color4) {
+  Sim* s = (Sim*)h;
+  for (uint64_t k = 0; k < count; k++) {
+    Particle& q = s->p[first + k];
+    const int64_t* o = q29 + k * 29;
+    q.pos = v3(R(o[0]), R(o[1]), R(o[2]));
+    q.vel = v3(R(o[3]), R(o[4]), R(o[5]));
+    for (int c = 0; c < 3; c++)
+      for (int r = 0; r < 3; r++) {
+        el(q.F.c[c], r) = R(o[6 + c * 3 + r]);
+        el(q.stress.c[c], r) = R(o[15 + c * 3 + r]);
+      }
+    q.mass = R(o[24]), q.volume = R(o[25]), q.plastic = R(o[26]), q.temperature = R(o[27]);
+    q.accum = R(o[28]);
+    q.active = flags3[k * 3] != 0, q.fractured = flags3[k * 3 + 1] != 0, q.num_connections = flags3[k * 3 + 2];
+    q.parent_id = ids9[k * 9];
+    for (int c = 0; c < 8; c++) q.connected[c] = ids9[k * 9 + 1 + c];
+    memcpy(q.color, color4 + k * 4, 16);
+  }
+}
 void fx_sim_state(void* h, uint64_t* step_count, double* time, uint64_t* next_id, uint32_t* n_events,
                   int64_t* hardening, int64_t* fracture_strain, uint32_t* fracture_on) {
   Sim* s = (Sim*)h;
@@ -784,6 +815,47 @@ void fx_sim_diag(void* h, int64_t* out3) {
   }
   out3[0] = m.raw, out3[1] = ke.raw, out3[2] = se.raw;
 }
+// active_particles, fractured_count, total_mass, average_temperature,
+// kinetic_energy, total_strain_energy (mpm.rs:744-778), serial folds
+void fx_sim_stats(void* h, int64_t* out6) {
+  Sim* s = (Sim*)h;
+  int64_t act = 0, frac = 0;
+  Fx m = ZERO, ke = ZERO, se = ZERO, t = ZERO;
+  for (const Particle& q : s->p) {
+    if (q.fractured) frac++;
+    if (!q.active) continue;
+    act++;
+    m = add(m, q.mass);
+    ke = add(ke, mul(mul(FP_HALF, q.mass), len2(q.vel)));
+    se = add(se, q.accum);
+    t = add(t, q.temperature);
+  }
+  out6[0] = act, out6[1] = frac, out6[2] = m.raw;
+  out6[3] = act == 0 ? from_f32(293.0f).raw : div(t, from_i32((int32_t)act)).raw;
+  out6[4] = ke.raw, out6[5] = se.raw;
+}
+// snapshots (mpm.rs:718-742): 13 floats per active particle (position3,
+// velocity3, stress magnitude, plastic strain, temperature, color4) and the
+// fractured flag; returns the number written
+uint64_t fx_sim_snapshots(void* h, float* out13, uint8_t* fractured) {
+  Sim* s = (Sim*)h;
+  uint64_t k = 0;
+  for (const Particle& q : s->p) {
+    if (!q.active) continue;
+    const M3& st = q.stress;
+    Fx mag = fsqrt(add(add(add(mul(st.c[0].x, st.c[0].x), mul(st.c[1].y, st.c[1].y)), mul(st.c[2].z, st.c[2].z)),
+                       mul(FP_TWO, add(add(mul(st.c[0].y, st.c[0].y), mul(st.c[0].z, st.c[0].z)), mul(st.c[1].z, st.c[1].z)))));
+    float* o = out13 + k * 13;
+    o[0] = to_f32(q.pos.x), o[1] = to_f32(q.pos.y), o[2] = to_f32(q.pos.z);
+    o[3] = to_f32(q.vel.x), o[4] = to_f32(q.vel.y), o[5] = to_f32(q.vel.z);
+    o[6] = to_f32(mag), o[7] = to_f32(q.plastic), o[8] = to_f32(q.temperature);
+    memcpy(o + 9, q.color, 16);
+    fractured[k] = q.fractured;
+    k++;
+  }
+  return k;
+}
+uint64_t fx_sim_scatter_bound(void* h) { return ((Sim*)h)->max_bound; }
 // Scalar FixedPoint operations for the unit tests.
 int64_t fx_op(int op, int64_t a, int64_t b) {
   build_sin_table();

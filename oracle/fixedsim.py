@@ -60,6 +60,14 @@ def lib():
         L.fx_sim_events.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_float),
                                     C.POINTER(C.c_float), C.POINTER(C.c_uint32)]
         L.fx_sim_diag.argtypes = [P, C.POINTER(C.c_int64)]
+        L.fx_sim_write.argtypes = [P, C.c_uint64, C.c_uint64, C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_uint8), C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_float)]
+        L.fx_sim_scatter_bound.restype = C.c_uint64
+        L.fx_sim_scatter_bound.argtypes = [P]
+        L.fx_sim_stats.argtypes = [P, C.POINTER(C.c_int64)]
+        L.fx_sim_snapshots.restype = C.c_uint64
+        L.fx_sim_snapshots.argtypes = [P, C.POINTER(C.c_float), C.POINTER(C.c_uint8)]
         L.fx_op.restype = C.c_int64
         L.fx_op.argtypes = [C.c_int, C.c_int64, C.c_int64]
         L.fx_from_f32.restype = C.c_int64
@@ -178,6 +186,16 @@ class Sim:
                           col.ctypes.data_as(C.POINTER(C.c_float)))
         return dict(q=q[:n], flags=fl[:n], ids=ids[:n], color=col[:n])
 
+    def write(self, first, q, flags, ids, color):
+        q = np.ascontiguousarray(q, np.int64).reshape(-1, 29)
+        fl = np.ascontiguousarray(flags, np.uint8).reshape(-1, 3)
+        ids = np.ascontiguousarray(ids, np.uint64).reshape(-1, 9)
+        col = np.ascontiguousarray(color, np.float32).reshape(-1, 4)
+        lib().fx_sim_write(self.h, first, len(q), q.ctypes.data_as(C.POINTER(C.c_int64)),
+                           fl.ctypes.data_as(C.POINTER(C.c_uint8)),
+                           ids.ctypes.data_as(C.POINTER(C.c_uint64)),
+                           col.ctypes.data_as(C.POINTER(C.c_float)))
+
     def state(self):
         sc, nid, ne, fo = C.c_uint64(), C.c_uint64(), C.c_uint32(), C.c_uint32()
         t = C.c_double()
@@ -203,6 +221,27 @@ class Sim:
         out = np.zeros(3, np.int64)
         lib().fx_sim_diag(self.h, out.ctypes.data_as(C.POINTER(C.c_int64)))
         return dict(total_mass=int(out[0]), kinetic_energy=int(out[1]), strain_energy=int(out[2]))
+
+    STATS = ("active_particles", "fractured_count", "total_mass", "average_temperature",
+             "kinetic_energy", "total_strain_energy")
+
+    def stats(self):
+        out = np.zeros(6, np.int64)
+        lib().fx_sim_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_int64)))
+        return {k: int(v) for k, v in zip(self.STATS, out)}
+
+    def snapshots(self):
+        n = self.count()
+        f = np.zeros((max(n, 1), 13), np.float32)
+        fr = np.zeros(max(n, 1), np.uint8)
+        k = lib().fx_sim_snapshots(self.h, f.ctypes.data_as(C.POINTER(C.c_float)),
+                                   fr.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return f[:k], fr[:k]
+
+    def scatter_bound(self):
+        """Largest per-substep sum of (|c| >> 24) + 1 over the nonzero grid
+        scatter contributions c."""
+        return int(lib().fx_sim_scatter_bound(self.h))
 
     def active_count(self):
         return int(self.read()["flags"][:, 0].sum())

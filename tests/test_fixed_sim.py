@@ -245,6 +245,9 @@ def test_sim_free_fall_one_substep_closed_form():
 # ---------------------------------------------------------------------------
 # GPU against the oracle
 # ---------------------------------------------------------------------------
+BOUND_LIMIT = 1 << 39  # sum of (|c| >> 24) + 1 below this: the atomic scatter is exact
+
+
 def _gpu_pair(cfg):
     import aemh
     return aemh.FixedSim(cfg), X.Sim(cfg)
@@ -259,34 +262,56 @@ def _compare(g, o, where):
     assert np.array_equal(dg["ids"], do["ids"]), where
     assert np.array_equal(dg["color"], do["color"]), where
     sg, so = g.state(), o.state()
-    for k in ("step_count", "next_id", "n_events", "hardening", "fracture_strain", "fracture_on"):
+    for k in ("step_count", "next_id", "n_events", "hardening", "fracture_strain", "fracture_on",
+              "time"):
         assert sg[k] == so[k], (where, k, sg[k], so[k])
-    assert sg["time"] == so["time"], where
+    assert g.stats() == o.stats(), where
+    ev_g, ev_o = g.events(), o.events()
+    for k in ev_o:
+        assert np.array_equal(ev_g[k], ev_o[k]), (where, k)
+    # the scatter bound is an exact integer sum below its limit
+    gb, ob = sg["max_scatter_bound"], o.scatter_bound()
+    assert (gb == ob) if ob < BOUND_LIMIT else (gb >= BOUND_LIMIT), (where, gb, ob)
 
 
-MODELS = [(X.ELASTIC, 0), (X.ELASTOPLASTIC, 0), (X.GRANULAR, X.from_f32(30.0)),
-          (X.FLUID, X.from_f32(0.01)), (X.SNOW, X.from_f32(0.05)), (X.BRITTLE, X.from_f32(0.02))]
+def _compare_snapshots(g, o):
+    snaps = g.snapshots()
+    f, fr = o.snapshots()
+    assert len(snaps) == len(f)
+    got = np.array([list(s.position) + list(s.velocity) + [s.stress_magnitude, s.plastic_strain,
+                    s.temperature] + list(s.color) for s in snaps], np.float32).reshape(-1, 13)
+    assert np.array_equal(got, f)
+    assert [s.fractured for s in snaps] == fr.tolist()
+
+
+# model, parameter, steps: each run stops before the reference's own
+# dynamics of that model reach the Q32.32 range (snow's SVD feeds A^T A back
+# into F, brittle fragments fracture again)
+MODELS = [(X.ELASTIC, 0, 8), (X.ELASTOPLASTIC, 0, 8), (X.GRANULAR, X.from_f32(30.0), 12),
+          (X.FLUID, X.from_f32(0.01), 12), (X.SNOW, X.from_f32(0.05), 3),
+          (X.BRITTLE, X.from_f32(0.02), 4)]
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("model,param", MODELS, ids=["elastic", "elastoplastic", "granular",
-                                                     "fluid", "snow", "brittle"])
-def test_gpu_fixed_models_bitexact(model, param):
-    cfg = X.small_config(model=model, model_param=param, particle_count=64, res=(8, 8, 8),
-                         cell=X.from_f32(0.25), substeps=2)
+@pytest.mark.parametrize("model,param,steps", MODELS,
+                         ids=["elastic", "elastoplastic", "granular", "fluid", "snow", "brittle"])
+@pytest.mark.parametrize("count,substeps", [(8, 1), (125, 2)], ids=["8x1", "125x2"])
+def test_gpu_fixed_models_bitexact(model, param, steps, count, substeps):
+    cfg = X.small_config(model=model, model_param=param, particle_count=count, substeps=substeps)
+    print(f"fixed model {model} particles {count} substeps {substeps}", flush=True)
     g, o = _gpu_pair(cfg)
-    for k in range(4):
-        g.step(3)
-        o.step(3)
-        _compare(g, o, f"model {model} step {3 * (k + 1)}")
-    dg, do = g.diag(), o.diag()
-    assert dg == do
+    for k in range(steps):
+        g.step(1)
+        o.step(1)
+        _compare(g, o, f"model {model} step {k + 1}")
+    _compare_snapshots(g, o)
     g.close()
 
 
 @pytest.mark.gpu
-def test_gpu_fixed_reference_config_add_remove_fracture():
+def test_gpu_fixed_fracture_add_remove_switch():
     cfg = X.small_config(enable_fracture=1, fracture_strain=X.from_f32(0.01))
+    print("fixed fracture/add/remove", flush=True)
     g, o = _gpu_pair(cfg)
     g.step(2)
     o.step(2)
@@ -297,26 +322,133 @@ def test_gpu_fixed_reference_config_add_remove_fracture():
     g.step(3)
     o.step(3)
     _compare(g, o, "fracture 5")
-    ev_g, ev_o = g.events(), o.events()
-    for k in ev_o:
-        assert np.array_equal(ev_g[k], ev_o[k]), k
+    assert o.state()["n_events"] > 0 and o.count() > 9
     pid = int(o.read()["ids"][2, 0])
-    assert g.remove_particle(pid) == o.remove_particle(pid) is True
+    assert g.remove_particle(pid) is True and o.remove_particle(pid) is True
+    assert g.remove_particle(10 ** 9) is False
     g.set_model(X.SNOW, X.from_f32(0.05))
     o.set_model(X.SNOW, X.from_f32(0.05))
-    g.step(2)
-    o.step(2)
+    g.set_fracture(False)
+    o.set_fracture(False)
+    g.step(1)
+    o.step(1)
     _compare(g, o, "after remove + snow")
+    _compare_snapshots(g, o)
     g.close()
 
 
 @pytest.mark.gpu
-def test_gpu_fixed_default_config_many_particles():
-    """MpmConfig::default: 10,000 particles (22^3 = 10,648 placed), 64^3 grid,
-    4 substeps per step."""
+def test_gpu_fixed_default_config():
+    """MpmConfig::default: 22^3 = 10,648 particles in one cell's span of a
+    64^3 grid, 4 substeps, all scattering into the same few nodes."""
     cfg = X.default_config()
+    print("fixed default config", flush=True)
     g, o = _gpu_pair(cfg)
     g.step(2)
     o.step(2)
     _compare(g, o, "default config")
     g.close()
+
+
+def _heavy(sims, stress, mass, shear=None):
+    """Assigns every particle a mass and a diagonal stress (and an
+    alternating xy shear), as a caller writing the public fields."""
+    d = sims[-1].read()
+    q = d["q"].copy()
+    q[:, 24] = X.from_f32(mass)
+    for k in (15, 19, 23):
+        q[:, k] = X.from_f32(stress)
+    if shear is not None:
+        q[::2, 16] = X.from_f32(shear)
+    for s_ in sims:
+        s_.write(0, q, d["flags"], d["ids"], d["color"])
+
+
+@pytest.mark.gpu
+def test_gpu_fixed_scatter_check_paths():
+    """Scatter exactness past the cheap bound.  Spread: 1,000 unit-mass
+    particles at stress 4e6 over a 16^3 grid, whose substep total passes the
+    limit while every cell stays far below it (the per-cell check alone).
+    Heavy: 27 particles of mass 1000 at stress 1e6 in one cell span, whose
+    cells can saturate (the serial rebuild, saturating adds in order)."""
+    print("fixed scatter check: spread", flush=True)
+    cfg = X.small_config(model=X.ELASTIC, particle_count=0, res=(16, 16, 16))
+    g, o = _gpu_pair(cfg)
+    rng = np.random.default_rng(11)
+    pos = np.array([[X.from_f32(v) for v in p] for p in rng.uniform(0.75, 7.25, (1000, 3))], np.int64)
+    zero = np.zeros((1000, 3), np.int64)
+    g.add_particles(pos, zero, np.full(1000, X.ONE, np.int64))
+    for i in range(1000):
+        o.add_particle(pos[i], zero[i], X.ONE)
+    _heavy((g, o), 4.0e6, 1.0)
+    for k in range(2):
+        g.step(1)
+        o.step(1)
+        _compare(g, o, f"spread step {k + 1}")
+    st = g.state()
+    assert o.scatter_bound() >= BOUND_LIMIT and st["checked_substeps"] > 0 and st["serial_substeps"] == 0
+    g.close()
+    print("fixed scatter check: heavy", flush=True)
+    cfg = X.small_config(model=X.ELASTIC, particle_count=27)
+    g, o = _gpu_pair(cfg)
+    _heavy((g, o), 1.0e6, 1000.0, shear=-3.0e5)
+    for k in range(3):
+        g.step(1)
+        o.step(1)
+        _compare(g, o, f"heavy step {k + 1}")
+    assert g.state()["serial_substeps"] > 0
+    g.close()
+
+
+@pytest.mark.gpu
+def test_gpu_fixed_snow_long():
+    """Snow for 12 steps: the reference's SVD returns the rotated A^T A as V,
+    so F = U S (A^T A)^T grows each substep and the stresses reach ~2e9."""
+    cfg = X.small_config(model=X.SNOW, model_param=X.from_f32(0.05), particle_count=125)
+    g, o = _gpu_pair(cfg)
+    for k in range(12):
+        g.step(1)
+        o.step(1)
+        _compare(g, o, f"snow step {k + 1}")
+    g.close()
+
+
+@pytest.mark.gpu
+def test_gpu_fixed_batch_add_write_and_empty():
+    """add_particles equals add_particle in a loop; writing records (a
+    caller assigning the public fields, e.g. deactivating) is honoured; an
+    empty simulation steps."""
+    import aemh
+    cfg = X.small_config(particle_count=0, res=(8, 8, 8), cell=X.from_f32(0.25))
+    g, o = _gpu_pair(cfg)
+    g.step(1)
+    o.step(1)
+    _compare(g, o, "empty")
+    rng = np.random.default_rng(5)
+    n = 300
+    pos = np.array([[X.from_f32(v) for v in p] for p in rng.uniform(0.3, 1.7, (n, 3))], np.int64)
+    vel = np.array([[X.from_f32(v) for v in p] for p in rng.uniform(-0.5, 0.5, (n, 3))], np.int64)
+    mass = np.array([X.from_f32(v) for v in rng.uniform(0.001, 0.01, n)], np.int64)
+    col = rng.uniform(0, 1, (n, 4)).astype(np.float32)
+    first = g.add_particles(pos, vel, mass, col)
+    ids = [o.add_particle(pos[i], vel[i], mass[i], tuple(col[i])) for i in range(n)]
+    assert first == ids[0] and ids == list(range(ids[0], ids[0] + n))
+    _compare(g, o, "batch add")
+    g.set_model(X.ELASTOPLASTIC, 0)
+    o.set_model(X.ELASTOPLASTIC, 0)
+    g.step(2)
+    o.step(2)
+    _compare(g, o, "batch add stepped")
+    d = o.read()
+    fl = d["flags"].copy()
+    fl[::7, 0] = 0
+    q = d["q"].copy()
+    q[::5, 3:6] = 0
+    for s in (g, o):
+        s.write(0, q, fl, d["ids"], d["color"])
+    g.step(2)
+    o.step(2)
+    _compare(g, o, "after write")
+    _compare_snapshots(g, o)
+    g.close()
+    assert isinstance(aemh.fixed_config_default(), aemh.FixedConfig)
