@@ -14,8 +14,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (ACCUM_F32, ACCUM_Q32_32, CLAMP_AUTO, DRUCKER_PRAGER, NEO_HOOKEAN,
-               NEWTONIAN_FLUID, SNOW, STORE_F32, Material, Scene, default_config, uniform_box)
+from . import (ACCUM_F32, ACCUM_Q32_32, CLAMP_AUTO, DRUCKER_PRAGER, FX_ELASTOPLASTIC,
+               NEO_HOOKEAN, NEWTONIAN_FLUID, SNOW, STORE_F32, Material, Scene, default_config,
+               fixed_config_default, uniform_box)
 
 SEED_BASE = 42
 
@@ -230,3 +231,46 @@ def engine_tiers(n_live=2_000_000, capacity=3_000_000, n_near=20_000, n_mid=490_
     mats = tier_materials()
     return TierWorld(name=f"engine_tiers_{n_live}", capacity=capacity, pos=pos[perm],
                      mat=rng.integers(0, len(mats), n_live).astype(np.uint16), materials=mats)
+
+
+def to_q32(v):
+    """FixedPoint::from_f32 (fixed_point.rs:22-25) of f32 values: f64 of the
+    f32 times 2^32, truncated toward zero (the values here are in range)."""
+    return (np.asarray(v, np.float32).astype(np.float64) * 4294967296.0).astype(np.int64)
+
+
+@dataclass
+class FixedScene:
+    name: str
+    config: object        # FixedConfig
+    pos: np.ndarray       # [n][3] raw Q32.32
+    mass: np.ndarray      # [n] raw Q32.32
+
+    @property
+    def n(self):
+        return self.pos.shape[0]
+
+
+def fixed_cloud(n=1 << 20, res=64, seed=7):
+    """The all-integer MpmSimulation at scale: n particles uniform
+    (xoshiro256**) in the middle half of a res^3 grid, with the material and
+    step of the reference's own MpmSimulation tests (small_config,
+    ae_physics/src/mpm.rs:957-968: cell 0.5, E 50, density 1, dt 1/60, one
+    substep) and its default elastoplastic model; mass = density x cube
+    volume / n.  The reference force term (stress x offset / cell^2, no
+    volume or dt) makes stiffer settings leave the grid within a few steps;
+    this one stays inside it for the first ten."""
+    cfg = fixed_config_default()
+    cfg.grid_resolution[:] = (res, res, res)
+    cfg.cell_size = int(to_q32(0.5))
+    cfg.youngs_modulus = int(to_q32(50.0))
+    cfg.density = int(to_q32(1.0))
+    cfg.dt = int(to_q32(1.0 / 60.0))
+    cfg.substeps = 1
+    cfg.particle_count = 0
+    cfg.material_model = FX_ELASTOPLASTIC
+    side = res * 0.25  # metres: half the grid's extent of res * 0.5
+    lo = np.full(3, res * 0.125, np.float32)
+    pos = uniform_box(SEED_BASE + seed, n, lo, lo + np.float32(side))
+    mass = np.full(n, to_q32(np.float32(side ** 3 / n)), np.int64)
+    return FixedScene(name=f"fixed_cloud_{n}_{res}", config=cfg, pos=to_q32(pos), mass=mass)

@@ -45,6 +45,11 @@ struct aemh_fixed {
   uint64_t* stage = nullptr;
   size_t stage_words = 0;
   uint32_t* find_idx = nullptr;
+  FxSortBufs sort{};
+  uint32_t sort_cap = 0;
+  bool sorted_p2g = true;
+  uint32_t since_sort = 0;    // substeps since the P2G order was sorted
+  bool order_valid = false;   // perm is a permutation of the current particles
   double time = 0.0;
   uint64_t step_count = 0, next_id = 1;
 };
@@ -54,6 +59,9 @@ namespace {
 using namespace q32;
 
 constexpr uint32_t kMaxParticles = 1u << 30;
+// P2G reuses its cell order this many substeps (particles move under a
+// cell per substep, so the runs of equal cells stay mostly whole)
+constexpr uint32_t kFxResortEvery = 4;
 
 aemh_status fxfail(aemh_fixed* s, aemh_status code, const char* fmt, ...) {
   char buf[512];
@@ -152,6 +160,7 @@ aemh_status ensure_cap(aemh_fixed* s, uint64_t need) {
 // uploads m records to [at, at + m)
 aemh_status put_records(aemh_fixed* s, uint32_t at, const Rec* recs, uint32_t m) {
   if (!m) return AEMH_OK;
+  s->order_valid = false;
   FXCHK(ensure_stage(s, (size_t)m * FX_NPLANES));
   FXTRY(s, hipMemcpyAsync(s->stage, recs, (size_t)m * sizeof(Rec), hipMemcpyHostToDevice, s->stream));
   FXTRY(s, fx_launch_put(s->st, at, s->stage, m, s->stream));
@@ -294,6 +303,40 @@ aemh_status fracture(aemh_fixed* s) {
   s->n = n0 + nfr;
   s->next_id += nfr;
   s->n_events += nev;
+  s->order_valid = false;
+  return AEMH_OK;
+}
+
+aemh_status ensure_sort(aemh_fixed* s, uint32_t n) {
+  if (!s->sort.bins) {  // bins follow the grid, fixed at create
+    s->sort.nbins = (uint32_t)((uint64_t)(s->cfg.grid_resolution[0] + 2) * (s->cfg.grid_resolution[1] + 2) *
+                               (s->cfg.grid_resolution[2] + 2)) + 1;
+    FXTRY(s, hipMalloc((void**)&s->sort.bins, (size_t)s->sort.nbins * 4));
+    FXTRY(s, hipMalloc((void**)&s->sort.cursor, (size_t)s->sort.nbins * 4));
+    s->sort.temp_bytes = std::max<size_t>(fx_sort_temp_bytes(s->sort.nbins), 256);
+    FXTRY(s, hipMalloc(&s->sort.temp, s->sort.temp_bytes));
+  }
+  if (n <= s->sort_cap) return AEMH_OK;
+  s->order_valid = false;
+  const uint32_t nc = std::max<uint32_t>(n, std::max<uint32_t>(1024, s->sort_cap * 2));
+  for (int k = 0; k < 2; ++k) {
+    void* old[] = {s->sort.keys[k], s->sort.rec[k], s->sort.perm[k]};
+    for (void* b : old)
+      if (b) FXTRY(s, hipFree(b));
+    s->sort.keys[k] = nullptr;
+    s->sort.rec[k] = nullptr;
+    s->sort.perm[k] = nullptr;
+  }
+  if (s->sort.rank) FXTRY(s, hipFree(s->sort.rank));
+  s->sort.rank = nullptr;
+  s->sort_cap = 0;
+  for (int k = 0; k < 2; ++k) {
+    FXTRY(s, hipMalloc((void**)&s->sort.keys[k], (size_t)nc * 4));
+    FXTRY(s, hipMalloc((void**)&s->sort.rec[k], (size_t)nc * kFxRecWords * 8));
+    FXTRY(s, hipMalloc((void**)&s->sort.perm[k], (size_t)nc * 4));
+  }
+  FXTRY(s, hipMalloc((void**)&s->sort.rank, (size_t)nc * 4));
+  s->sort_cap = nc;
   return AEMH_OK;
 }
 
@@ -329,9 +372,19 @@ aemh_status substep(aemh_fixed* s, int64_t sdt) {
   sp.crit_s = add(ONE, from_f32(7.5e-3f));
   sp.n = s->n;
   FXTRY(s, hipMemsetAsync(s->grid, 0, (size_t)s->ncells * 32, s->stream));
-  FXTRY(s, fx_launch_p2g(s->st, sp, s->grid, s->bound, s->cell_bound, s->ncells, s->counters, s->stream));
+  int resort = FX_ORDER_KEEP;
+  if (s->sorted_p2g) {
+    FXCHK(ensure_sort(s, s->n));
+    resort = !s->order_valid ? FX_ORDER_FROM_PLANES
+                             : (s->since_sort >= kFxResortEvery ? FX_ORDER_RESORT : FX_ORDER_KEEP);
+    s->since_sort = resort != FX_ORDER_KEEP ? 1 : s->since_sort + 1;
+    s->order_valid = true;
+  }
+  FXTRY(s, fx_launch_p2g(s->st, sp, s->grid, s->bound, s->cell_bound, s->ncells, s->counters,
+                         s->sorted_p2g ? &s->sort : nullptr, resort, s->stream));
   FXTRY(s, fx_launch_grid(s->grid, s->ncells, sp.gdt, s->bound, s->stream));
-  FXTRY(s, fx_launch_g2p((int)model, s->st, sp, s->grid, s->sin_tab, s->stream));
+  FXTRY(s, fx_launch_g2p((int)model, s->st, sp, s->grid, s->sin_tab, s->sorted_p2g ? &s->sort : nullptr,
+                         s->stream));
   if (model == AEMH_FX_BRITTLE) {  // :462-465
     c.fracture_strain = c.model_param;
     c.enable_fracture = 1;
@@ -383,7 +436,9 @@ aemh_status aemh_fixed_create(aemh_fixed** out, int device, const aemh_fixed_con
   const uint64_t cells = (uint64_t)cfg->grid_resolution[0] * cfg->grid_resolution[1] * cfg->grid_resolution[2];
   if (cells == 0 || cells > (1ull << 31) || cfg->grid_resolution[0] > INT32_MAX ||
       cfg->grid_resolution[1] > INT32_MAX || cfg->grid_resolution[2] > INT32_MAX || cfg->substeps > INT32_MAX ||
-      cfg->material_model > AEMH_FX_SNOW)
+      cfg->material_model > AEMH_FX_SNOW ||
+      (uint64_t)(cfg->grid_resolution[0] + 2ull) * (cfg->grid_resolution[1] + 2ull) * (cfg->grid_resolution[2] + 2ull) >=
+          (1ull << 32) - 128)  // cell keys stay clear of the idle-lane keys near 2^32
     return AEMH_ERR_ARG;
   aemh_fixed* s = new aemh_fixed();
   s->device = device;
@@ -427,7 +482,9 @@ void aemh_fixed_destroy(aemh_fixed* s) {
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   void* bufs[] = {s->st.base, s->grid, s->sin_tab, s->frag_off, s->bound, s->cell_bound, s->counters,
-                  s->key, s->off, s->strain, s->scan_tmp, s->events, s->acc, s->stage, s->find_idx};
+                  s->key, s->off, s->strain, s->scan_tmp, s->events, s->acc, s->stage, s->find_idx,
+                  s->sort.keys[0], s->sort.keys[1], s->sort.rec[0], s->sort.rec[1], s->sort.perm[0],
+                  s->sort.perm[1], s->sort.rank, s->sort.bins, s->sort.cursor, s->sort.temp};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (s->total_h) (void)hipHostFree(s->total_h);
@@ -495,6 +552,7 @@ aemh_status aemh_fixed_remove_particle(aemh_fixed* s, uint64_t id, int* removed)
     FXTRY(s, hipStreamSynchronize(s->stream));
   }
   s->n -= 1;
+  s->order_valid = false;
   if (removed) *removed = 1;
   return AEMH_OK;
 }

@@ -85,13 +85,45 @@ constexpr int kFxFragTable = 9;                        // fragments per event: 1
 // tripped flag, the maximum over finished substeps, any substep tripped;
 // the grid kernel folds and clears the first two), cell_bound[ncells]
 // zeroed, counters[2] (substeps checked per cell, rebuilt serially)
+// The order P2G visits particles in: by base cell on the (res + 2)^3
+// lattice (nbins = that size + 1, the last bin holding particles that
+// scatter nothing; below 2^32).  P2G reads packed records in that order:
+// position, velocity, stress, mass (kFxRecWords int64), written by the
+// sort and then by each G2P at rank[i] together with the particle's new
+// key.  Records, keys and perm (slot -> particle) are double-buffered; cur
+// is the live copy.
+constexpr int kFxRecWords = 16;
+struct FxSortBufs {
+  uint32_t* keys[2];
+  int64_t* rec[2];
+  uint32_t* perm[2];
+  uint32_t* rank;    // particle -> slot
+  uint32_t* bins;    // nbins counts
+  uint32_t* cursor;  // nbins running offsets
+  uint32_t nbins;
+  int cur;
+  void* temp;
+  size_t temp_bytes;
+};
+enum { FX_ORDER_KEEP = 0, FX_ORDER_RESORT = 1, FX_ORDER_FROM_PLANES = 2 };
+struct FxOrder {  // what G2P needs to refresh the records (rec == nullptr: nothing)
+  const uint32_t* rank;
+  int64_t* rec;
+  uint32_t* keys;
+  uint32_t none;
+};
+size_t fx_sort_temp_bytes(uint32_t nbins);
+// sort == nullptr scatters in particle order with one atomic per value;
+// else in the sort's order: kept (FX_ORDER_KEEP), re-sorted from the live
+// records (FX_ORDER_RESORT), or rebuilt from the particle planes
+// (FX_ORDER_FROM_PLANES: required after the particle set changed)
 hipError_t fx_launch_p2g(const FxStore& st, const FxSubstep& sp, int64_t* grid,
                          unsigned long long* bound, unsigned long long* cell_bound, uint32_t ncells,
-                         unsigned long long* counters, hipStream_t s);
+                         unsigned long long* counters, FxSortBufs* sort, int resort, hipStream_t s);
 hipError_t fx_launch_grid(int64_t* grid, uint32_t ncells, const int64_t gdt[3], unsigned long long* bound,
                           hipStream_t s);
 hipError_t fx_launch_g2p(int model, const FxStore& st, const FxSubstep& sp, const int64_t* grid,
-                         const int64_t* sin_tab, hipStream_t s);
+                         const int64_t* sin_tab, const FxSortBufs* sort, hipStream_t s);
 hipError_t fx_launch_frac_mark(const FxStore& st, uint32_t n, int64_t threshold, uint64_t* key,
                                int64_t* strain, hipStream_t s);
 size_t fx_scan_temp_bytes(uint32_t n);

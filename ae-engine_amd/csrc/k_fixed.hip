@@ -78,6 +78,9 @@ __device__ inline void weights(int64_t fx, int64_t w[3]) {
   for (int d = -1; d <= 1; ++d) w[d + 1] = bspline(sub(fx, from_int(d)));
 }
 
+// v[d + 1] for d in {-1, 0, 1} by selects (stays in registers in a loop)
+__device__ inline int64_t sel3(const int64_t (&v)[3], int d) { return d < 0 ? v[0] : (d == 0 ? v[1] : v[2]); }
+
 __device__ inline bool in_grid(int32_t x, int32_t y, int32_t z, const int32_t* res) {
   return x >= 0 && x < res[0] && y >= 0 && y < res[1] && z >= 0 && z < res[2];
 }
@@ -139,6 +142,210 @@ __global__ __launch_bounds__(kFxBlock) void k_fx_p2g(FxStore st, FxSubstep sp, i
             bsum += bound_of(v[k]);
           }
       });
+  }
+  push_bound(bsum, bound, bound + 1);
+}
+
+// Aggregation key of a particle: its base cell on the (res + 2)^3 lattice
+// that also holds the bases one cell outside the grid (their +1 nodes are
+// inside); `none` for inactive particles and bases further out, which
+// scatter nothing.
+__device__ inline uint32_t agg_key(const PAt& P, const FxSubstep& sp, uint32_t none) {
+  if (!(P(FX_FLAGS) & 1)) return none;
+  int32_t b[3];
+  for (int a = 0; a < 3; ++a) {
+    int64_t fx;
+    locate(P(FX_POS + a), sp.inv_cell, fx, b[a]);
+    if (b[a] < -1 || b[a] > sp.res[a]) return none;
+  }
+  const uint32_t ex = (uint32_t)sp.res[0] + 2, ey = (uint32_t)sp.res[1] + 2;
+  return (uint32_t)(b[0] + 1) + ex * ((uint32_t)(b[1] + 1) + ey * (uint32_t)(b[2] + 1));
+}
+
+// Counting sort of the P2G order by base cell (k_fx_bin_count, a scan of
+// the bins, k_fx_bin_scatter).  Not stable: particles of one cell land in
+// any order, which the exact integer scatter does not see.  The source is
+// either the particle planes in particle order (after the particle set
+// changed), keys computed here, or the previous order's records and keys.
+// Runs of equal keys within a wave (a sorted source has long ones): the
+// run's first lane and each lane's position in its run.  Every lane calls
+// it (lanes past the end pass a key of their own that matches no other).
+struct WaveRun {
+  bool head;
+  uint32_t pos, len;
+};
+__device__ inline WaveRun wave_run(uint32_t key) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t prev = __shfl_up(key, 1);
+  const bool head = lane == 0 || prev != key;
+  uint32_t rid = head ? lane : 0u;  // lane of the run's head
+  for (int k = 0; k < 6; ++k) {
+    const uint32_t o = __shfl_up(rid, 1u << k);
+    if (lane >= (1u << k) && o > rid) rid = o;
+  }
+  // the run ends where the next head starts: lanes of the run count up to it
+  const uint64_t heads = __ballot(head);
+  const uint64_t later = heads & ~((2ull << lane) - 1);  // heads after this lane
+  const uint32_t end = later ? (uint32_t)__builtin_ctzll(later) : 64u;
+  return WaveRun{head, lane - rid, end - rid};
+}
+
+__global__ __launch_bounds__(kFxBlock) void k_fx_bin_count(FxStore st, FxSubstep sp, uint32_t none, bool from_planes,
+                                                            uint32_t* __restrict__ keys, uint32_t* __restrict__ bins) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = u < sp.n;
+  uint32_t key = 0xffffffffu - (threadIdx.x & 63);  // distinct per idle lane
+  if (in) {
+    if (from_planes) {
+      key = agg_key(PAt{st.base, st.stride, u}, sp, none);
+      keys[u] = key;
+    } else {
+      key = keys[u];
+    }
+  }
+  const WaveRun r = wave_run(key);
+  if (in && r.head) atomicAdd(bins + key, r.len);
+}
+
+__global__ __launch_bounds__(kFxBlock) void k_fx_bin_scatter(FxStore st, uint32_t n, bool from_planes,
+                                                              const uint32_t* __restrict__ keys,
+                                                              const int64_t* __restrict__ rec,
+                                                              const uint32_t* __restrict__ perm,
+                                                              uint32_t* __restrict__ cursor,
+                                                              uint32_t* __restrict__ keys_out,
+                                                              int64_t* __restrict__ rec_out,
+                                                              uint32_t* __restrict__ perm_out,
+                                                              uint32_t* __restrict__ rank) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = u < n;
+  const uint32_t key = in ? keys[u] : 0xffffffffu - (threadIdx.x & 63);
+  const WaveRun run = wave_run(key);
+  uint32_t base = 0;
+  if (in && run.head) base = atomicAdd(cursor + key, run.len);
+  // the head's slot, from the lane that holds it
+  base = __shfl(base, (int)((threadIdx.x & 63) - run.pos));
+  if (!in) return;
+  const uint32_t v = base + run.pos;
+  int64_t* r = rec_out + (size_t)v * kFxRecWords;
+  uint32_t i;
+  if (from_planes) {
+    i = u;
+    const PAt P{st.base, st.stride, u};
+    for (int k = 0; k < 6; ++k) r[k] = P(FX_POS + k);  // position, velocity
+    for (int k = 0; k < 9; ++k) r[6 + k] = P(FX_STRESS + k);
+    r[15] = P(FX_MASS);
+  } else {
+    i = perm[u];
+    const int64_t* q = rec + (size_t)u * kFxRecWords;
+    for (int k = 0; k < kFxRecWords; ++k) r[k] = q[k];
+  }
+  keys_out[v] = key;
+  perm_out[v] = i;
+  rank[i] = v;
+}
+
+// particles_to_grid over the particles in (roughly) key order, reading
+// their packed records: lanes of a wave in one run of equal base cells
+// share all 27 nodes, so each node value is summed over the run (a
+// segmented reduction by shuffles) and only the run's first lane adds it to
+// the grid.  Runs are maximal stretches of equal adjacent keys, named by
+// their first lane, so an order left over from an earlier sort (equal keys
+// split into several runs) is still summed exactly.  The sums are wrapping
+// int64 like the atomics, and the bound still counts every particle's own
+// contributions.
+__global__ __launch_bounds__(kFxBlock) void k_fx_p2g_sorted(FxSubstep sp, const uint32_t* __restrict__ skeys,
+                                                             const int64_t* __restrict__ rec, uint32_t none,
+                                                             int64_t* __restrict__ grid,
+                                                             unsigned long long* __restrict__ bound) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t key = t < sp.n ? skeys[t] : none;
+  // the shuffles run on every lane: a lane masked off (by a short-circuit,
+  // say) would hand its neighbour an undefined value
+  const uint32_t prev = __shfl_up(key, 1);
+  const bool head = lane == 0 || prev != key;
+  // run id: the lane of the run's head (inclusive max-scan of head lanes)
+  uint32_t rid = head ? lane : 0u;
+  for (int k = 0; k < 6; ++k) {
+    const uint32_t o = __shfl_up(rid, 1u << k);
+    if (lane >= (1u << k) && o > rid) rid = o;
+  }
+  // same bit k: lane + 2^k is in this lane's run (so is every lane between)
+  uint32_t same = 0;
+  for (int k = 0; k < 6; ++k) {
+    const uint32_t o = 1u << k;
+    const uint32_t other = __shfl_down(rid, o);
+    if (lane + o < 64 && other == rid) same |= 1u << k;
+  }
+  const bool live = key != none;
+  int64_t pos[3] = {0, 0, 0}, vel[3] = {0, 0, 0}, fx[3] = {0, 0, 0}, w[3][3];
+  int32_t b[3] = {0, 0, 0};
+  Mat stress = mat_zero();
+  int64_t mass = 0;
+  if (live) {
+    const int64_t* r = rec + (size_t)t * kFxRecWords;
+    for (int a = 0; a < 3; ++a) {
+      pos[a] = r[a];
+      vel[a] = r[3 + a];
+      locate(pos[a], sp.inv_cell, fx[a], b[a]);
+      weights(fx[a], w[a]);
+    }
+    for (int k = 0; k < 9; ++k) stress.m[k] = r[6 + k];
+    mass = r[15];
+  }
+  // Component a of node (dx, dy, dz) minus the position depends on one
+  // axis offset only, and so does each product of the stress-times-offset
+  // sums (row r, column a): term[a][r][d] = stress(r, a) * rel_a(d).
+  int64_t term[3][3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int d = 0; d < 3; ++d) {
+      const int64_t rel = sub(mul(from_int(b[a] + d - 1), sp.cell), pos[a]);
+      for (int r = 0; r < 3; ++r) term[a][r][d] = mul(stress.at(r, a), rel);
+    }
+  // every lane of a run that reaches a head has the same base; a wave with
+  // no repeated key skips the reductions
+  const bool any_run = __any(same != 0);
+  uint64_t bsum = 0;
+  for (int dx = -1; dx <= 1; ++dx) {
+    const int64_t wx = sel3(w[0], dx);
+    int64_t tx[3];
+    for (int r = 0; r < 3; ++r) tx[r] = sel3(term[0][r], dx);
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int64_t wy = sel3(w[1], dy);
+      int64_t ty[3];
+      for (int r = 0; r < 3; ++r) ty[r] = sel3(term[1][r], dy);
+      for (int dz = -1; dz <= 1; ++dz) {
+        const int32_t gx = b[0] + dx, gy = b[1] + dy, gz = b[2] + dz;
+        const bool node = live && in_grid(gx, gy, gz, sp.res);
+        int64_t v[4] = {0, 0, 0, 0};
+        if (node) {
+          const int64_t wt = mul_w(mul_w(wx, (uint32_t)wy), (uint32_t)sel3(w[2], dz));
+          if (wt >= EPS) {
+            int64_t sr[3];  // stress * rel, each row summed x, y, z (matvec)
+            for (int r = 0; r < 3; ++r) sr[r] = add(add(tx[r], ty[r]), sel3(term[2][r], dz));
+            v[0] = mul_w(mass, (uint32_t)wt);
+            for (int a = 0; a < 3; ++a)
+              v[1 + a] = mul_w(mul(mass, add(vel[a], mul(mul(sr[a], sp.inv_cell), sp.inv_cell))), (uint32_t)wt);
+            for (int k = 0; k < 4; ++k) bsum += bound_of(v[k]);
+          }
+        }
+        if (any_run) {
+          for (int k = 0; k < 4; ++k) {
+            uint64_t x = (uint64_t)v[k];
+            for (int q = 0; q < 6; ++q) {
+              const uint64_t o = __shfl_down(x, 1u << q);
+              if (same & (1u << q)) x += o;
+            }
+            v[k] = (int64_t)x;
+          }
+        }
+        if (node && head) {
+          int64_t* g = grid + cell_of(gx, gy, gz, sp.res) * 4;
+          for (int k = 0; k < 4; ++k)
+            if (v[k] != 0) atomicAdd(reinterpret_cast<unsigned long long*>(g + k), (unsigned long long)v[k]);
+        }
+      }
+    }
   }
   push_bound(bsum, bound, bound + 1);
 }
@@ -232,7 +439,7 @@ __device__ inline Mat plastic_return(const Mat& stress, int64_t h, const Mat& de
 // constitutive model (:456-639) of one particle
 template <int MODEL>
 __global__ __launch_bounds__(kFxBlock) void k_fx_g2p(FxStore st, FxSubstep sp, const int64_t* __restrict__ grid,
-                                                      const int64_t* __restrict__ tab) {
+                                                      const int64_t* __restrict__ tab, FxOrder ord) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= sp.n) return;
   const PAt P{st.base, st.stride, i};
@@ -251,15 +458,16 @@ __global__ __launch_bounds__(kFxBlock) void k_fx_g2p(FxStore st, FxSubstep sp, c
       for (int dz = -1; dz <= 1; ++dz) {
         const int32_t gx = b[0] + dx, gy = b[1] + dy, gz = b[2] + dz;
         if (!in_grid(gx, gy, gz, sp.res)) continue;
-        const int64_t wt = mul(mul(w[0][dx + 1], w[1][dy + 1]), w[2][dz + 1]);
-        if (wt < EPS) continue;
+        const int64_t wt = mul_w(w[0][dx + 1], (uint32_t)w[1][dy + 1]);
+        const int64_t wt3 = mul_w(wt, (uint32_t)w[2][dz + 1]);
+        if (wt3 < EPS) continue;
         const int64_t* g = grid + cell_of(gx, gy, gz, sp.res) * 4;
         if (g[0] < EPS) continue;
         const int64_t gv[3] = {g[1], g[2], g[3]};
         const int64_t gd[3] = {sub(from_int(dx), fx[0]), sub(from_int(dy), fx[1]), sub(from_int(dz), fx[2])};
         for (int a = 0; a < 3; ++a) {
-          nv[a] = add(nv[a], mul(gv[a], wt));
-          const int64_t gw = mul(gv[a], wt);
+          const int64_t gw = mul_w(gv[a], (uint32_t)wt3);  // grid_vel * weight
+          nv[a] = add(nv[a], gw);
           for (int c = 0; c < 3; ++c) vg.m[a * 3 + c] = add(vg.m[a * 3 + c], mul(mul(gw, gd[c]), sp.inv_cell));
         }
       }
@@ -350,6 +558,24 @@ __global__ __launch_bounds__(kFxBlock) void k_fx_g2p(FxStore st, FxSubstep sp, c
     out[(FX_STRESS + k) * S] = stress.m[k];
   }
   out[FX_PLASTIC * S] = plastic;
+  if (ord.rec) {  // the next P2G's record and key, at this particle's slot in its order
+    const uint32_t t = ord.rank[i];
+    int64_t* r = ord.rec + (size_t)t * kFxRecWords;
+    int32_t nb[3];
+    bool inside = true;
+    for (int a = 0; a < 3; ++a) {
+      const int64_t np = add(pos[a], mul(nv[a], sp.sdt));
+      r[a] = np;
+      r[3 + a] = vel[a];
+      int64_t f;
+      locate(np, sp.inv_cell, f, nb[a]);
+      inside &= nb[a] >= -1 && nb[a] <= sp.res[a];
+    }
+    for (int k = 0; k < 9; ++k) r[6 + k] = stress.m[k];
+    r[15] = P(FX_MASS);
+    const uint32_t ex = (uint32_t)sp.res[0] + 2, ey = (uint32_t)sp.res[1] + 2;
+    ord.keys[t] = inside ? (uint32_t)(nb[0] + 1) + ex * ((uint32_t)(nb[1] + 1) + ey * (uint32_t)(nb[2] + 1)) : ord.none;
+  }
 }
 
 // handle_fracture (mpm.rs:641-716), first pass: which particles break and
@@ -538,11 +764,37 @@ inline unsigned blocks_for(size_t n) { return (unsigned)((n + kFxBlock - 1) / kF
 
 }  // namespace
 
+size_t fx_sort_temp_bytes(uint32_t nbins) {
+  size_t bytes = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)nbins);
+  return bytes;
+}
+
 hipError_t fx_launch_p2g(const FxStore& st, const FxSubstep& sp, int64_t* grid, unsigned long long* bound,
                          unsigned long long* cell_bound, uint32_t ncells, unsigned long long* counters,
-                         hipStream_t s) {
+                         FxSortBufs* sort, int resort, hipStream_t s) {
   if (!sp.n) return hipSuccess;
-  k_fx_p2g<<<blocks_for(sp.n), kFxBlock, 0, s>>>(st, sp, grid, bound);
+  if (sort) {
+    const uint32_t none = sort->nbins - 1;
+    if (resort != FX_ORDER_KEEP) {
+      const bool planes = resort == FX_ORDER_FROM_PLANES;
+      const int src = planes ? 1 - sort->cur : sort->cur, dst = 1 - src;
+      hipError_t e = hipMemsetAsync(sort->bins, 0, (size_t)sort->nbins * 4, s);
+      if (e != hipSuccess) return e;
+      k_fx_bin_count<<<blocks_for(sp.n), kFxBlock, 0, s>>>(st, sp, none, planes, sort->keys[src], sort->bins);
+      size_t bytes = sort->temp_bytes;
+      e = hipcub::DeviceScan::ExclusiveSum(sort->temp, bytes, sort->bins, sort->cursor, (int)sort->nbins, s);
+      if (e != hipSuccess) return e;
+      k_fx_bin_scatter<<<blocks_for(sp.n), kFxBlock, 0, s>>>(st, sp.n, planes, sort->keys[src], sort->rec[src],
+                                                              sort->perm[src], sort->cursor, sort->keys[dst],
+                                                              sort->rec[dst], sort->perm[dst], sort->rank);
+      sort->cur = dst;
+    }
+    k_fx_p2g_sorted<<<blocks_for(sp.n), kFxBlock, 0, s>>>(sp, sort->keys[sort->cur], sort->rec[sort->cur], none,
+                                                            grid, bound);
+  } else {
+    k_fx_p2g<<<blocks_for(sp.n), kFxBlock, 0, s>>>(st, sp, grid, bound);
+  }
   k_fx_cell_bound<<<blocks_for(sp.n), kFxBlock, 0, s>>>(st, sp, bound, cell_bound);
   k_fx_p2g_settle<<<1, 256, 0, s>>>(st, sp, grid, bound, cell_bound, ncells, counters);
   return hipGetLastError();
@@ -555,16 +807,18 @@ hipError_t fx_launch_grid(int64_t* grid, uint32_t ncells, const int64_t gdt[3], 
 }
 
 hipError_t fx_launch_g2p(int model, const FxStore& st, const FxSubstep& sp, const int64_t* grid,
-                         const int64_t* sin_tab, hipStream_t s) {
+                         const int64_t* sin_tab, const FxSortBufs* sort, hipStream_t s) {
   if (!sp.n) return hipSuccess;
   const unsigned g = blocks_for(sp.n);
+  FxOrder ord{nullptr, nullptr, nullptr, 0};
+  if (sort) ord = FxOrder{sort->rank, sort->rec[sort->cur], sort->keys[sort->cur], sort->nbins - 1};
   switch (model) {
-    case FXM_ELASTIC: k_fx_g2p<FXM_ELASTIC><<<g, kFxBlock, 0, s>>>(st, sp, grid, sin_tab); break;
-    case FXM_ELASTOPLASTIC: k_fx_g2p<FXM_ELASTOPLASTIC><<<g, kFxBlock, 0, s>>>(st, sp, grid, sin_tab); break;
-    case FXM_GRANULAR: k_fx_g2p<FXM_GRANULAR><<<g, kFxBlock, 0, s>>>(st, sp, grid, sin_tab); break;
-    case FXM_BRITTLE: k_fx_g2p<FXM_BRITTLE><<<g, kFxBlock, 0, s>>>(st, sp, grid, sin_tab); break;
-    case FXM_FLUID: k_fx_g2p<FXM_FLUID><<<g, kFxBlock, 0, s>>>(st, sp, grid, sin_tab); break;
-    case FXM_SNOW: k_fx_g2p<FXM_SNOW><<<g, kFxBlock, 0, s>>>(st, sp, grid, sin_tab); break;
+    case FXM_ELASTIC: k_fx_g2p<FXM_ELASTIC><<<g, kFxBlock, 0, s>>>(st, sp, grid, sin_tab, ord); break;
+    case FXM_ELASTOPLASTIC: k_fx_g2p<FXM_ELASTOPLASTIC><<<g, kFxBlock, 0, s>>>(st, sp, grid, sin_tab, ord); break;
+    case FXM_GRANULAR: k_fx_g2p<FXM_GRANULAR><<<g, kFxBlock, 0, s>>>(st, sp, grid, sin_tab, ord); break;
+    case FXM_BRITTLE: k_fx_g2p<FXM_BRITTLE><<<g, kFxBlock, 0, s>>>(st, sp, grid, sin_tab, ord); break;
+    case FXM_FLUID: k_fx_g2p<FXM_FLUID><<<g, kFxBlock, 0, s>>>(st, sp, grid, sin_tab, ord); break;
+    case FXM_SNOW: k_fx_g2p<FXM_SNOW><<<g, kFxBlock, 0, s>>>(st, sp, grid, sin_tab, ord); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

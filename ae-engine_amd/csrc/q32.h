@@ -60,12 +60,34 @@ Q32_FN int64_t neg(int64_t a) { return a == MINV ? MAXV : -a; }  // saturating_n
 Q32_FN int64_t qabs(int64_t a) { return a < 0 ? neg(a) : a; }    // saturating_abs
 Q32_FN uint64_t mag(int64_t a) { return a < 0 ? 0 - (uint64_t)a : (uint64_t)a; }
 
-// (a * b) / 2^32 truncated toward zero, low 64 bits
+// (a * b) / 2^32 truncated toward zero, low 64 bits.  With a = ah 2^32 + al
+// (ah signed, al unsigned 32-bit) and b likewise, the product is
+// ah bh 2^64 + (ah bl + al bh) 2^32 + al bl, so its floor over 2^32 modulo
+// 2^64 needs only al bl, the two cross products and the low half of ah bh;
+// a negative product with a nonzero low word (al bl mod 2^32) rounds up.
 Q32_FN int64_t mul(int64_t a, int64_t b) {
-  const i128 p = (i128)a * (i128)b;
-  const int64_t q = (int64_t)(p >> 32);
-  return q + (int64_t)((p < 0) & (((uint64_t)p & 0xffffffffu) != 0));
+  const uint32_t al = (uint32_t)a, bl = (uint32_t)b;
+  const uint32_t ahu = (uint32_t)((uint64_t)a >> 32), bhu = (uint32_t)((uint64_t)b >> 32);
+  const uint64_t ll = (uint64_t)al * bl;
+  // ah bl and al bh as unsigned products; a negative ah (bh) contributes
+  // -2^32 bl (-2^32 al), folded into the high word below
+  const uint64_t mid = (ll >> 32) + (uint64_t)ahu * bl + (uint64_t)al * bhu;
+  const uint32_t hi = ahu * bhu - (a < 0 ? bl : 0u) - (b < 0 ? al : 0u);
+  const uint64_t q = mid + ((uint64_t)hi << 32);
+  return (int64_t)(q + (uint64_t)(((a ^ b) < 0) & ((uint32_t)ll != 0)));
 }
+// mul(x, w) for 0 <= w < 2^32 (a B-spline weight or a product of two: the
+// weights lie in [0, 0.75]): the same value as mul with fewer partial
+// products, since w has no high word
+Q32_FN int64_t mul_w(int64_t x, uint32_t w) {
+  const uint32_t xl = (uint32_t)x, xhu = (uint32_t)((uint64_t)x >> 32);
+  const uint64_t ll = (uint64_t)xl * w;
+  const uint64_t mid = (ll >> 32) + (uint64_t)xhu * w;
+  const uint64_t q = mid - (x < 0 ? (uint64_t)w << 32 : 0ull);
+  return (int64_t)(q + (uint64_t)((x < 0) & ((uint32_t)ll != 0)));
+}
+// a * b as i128 (never overflows)
+Q32_FN i128 wide_mul(int64_t a, int64_t b) { return (i128)a * (i128)b; }
 
 Q32_FN double wide_to_double(u128 n) {
   return (double)(uint64_t)(n >> 64) * 18446744073709551616.0 + (double)(uint64_t)n;
@@ -97,6 +119,25 @@ Q32_FN uint64_t udiv96(u128 n, uint64_t d) {
     }
   }
   return q;
+}
+
+// floor(n / d) as udiv96, the estimate from a reciprocal of d computed once
+// for several numerators (the integer correction makes it exact)
+Q32_FN uint64_t udiv96_rcp(u128 n, uint64_t d, double rcp) {
+  if ((n >> 50) < (u128)d) {
+    uint64_t q = (uint64_t)(wide_to_double(n) * rcp);
+    u128 prod = (u128)q * d;
+    while (prod > n) {
+      q -= 1;
+      prod -= d;
+    }
+    while (n - prod >= (u128)d) {
+      q += 1;
+      prod += d;
+    }
+    return q;
+  }
+  return udiv96(n, d);
 }
 
 // (a * 2^32) / b truncated toward zero, low 64 bits
@@ -226,7 +267,16 @@ Q32_FN Mat mat_scale(const Mat& a, int64_t s) {
 }
 Q32_FN Mat mat_div(const Mat& a, int64_t s) {
   Mat o;
-  for (int k = 0; k < 9; ++k) o.m[k] = div(a.m[k], s);
+  if (s == 0) {
+    for (int k = 0; k < 9; ++k) o.m[k] = div(a.m[k], s);
+    return o;
+  }
+  const uint64_t d = mag(s);
+  const double rcp = 1.0 / (double)d;
+  for (int k = 0; k < 9; ++k) {
+    const uint64_t q = udiv96_rcp((u128)mag(a.m[k]) << 32, d, rcp);
+    o.m[k] = (int64_t)(((a.m[k] < 0) != (s < 0)) ? 0 - q : q);
+  }
   return o;
 }
 // (a b)(r, c) = row r of a . column c of b (fixed_point.rs:810-830)
@@ -242,11 +292,9 @@ Q32_FN void matvec(const Mat& a, const int64_t v[3], int64_t o[3]) {
   for (int r = 0; r < 3; ++r) o[r] = add(add(mul(a.at(r, 0), v[0]), mul(a.at(r, 1), v[1])), mul(a.at(r, 2), v[2]));
 }
 
-// checked i128 pieces of the determinant; an overflowing piece is 0 (:711-729)
-Q32_FN i128 ck_mul(i128 a, i128 b) {
-  i128 r;
-  return __builtin_mul_overflow(a, b, &r) ? 0 : r;
-}
+// checked i128 pieces of the determinant; an overflowing piece is 0
+// (:711-729).  Products of two i64 always fit, so only the i64 x i128
+// products and the sums are checked.
 Q32_FN i128 ck_sub(i128 a, i128 b) {
   i128 r;
   return __builtin_sub_overflow(a, b, &r) ? 0 : r;
@@ -255,12 +303,26 @@ Q32_FN i128 ck_add(i128 a, i128 b) {
   i128 r;
   return __builtin_add_overflow(a, b, &r) ? 0 : r;
 }
+// a * x, or 0 when the product leaves the i128 range
+Q32_FN i128 ck_mul_wide(int64_t a, i128 x) {
+  const bool neg_r = (a < 0) != (x < 0);
+  const uint64_t ua = mag(a);
+  const u128 ux = x < 0 ? (u128)0 - (u128)x : (u128)x;
+  const u128 p0 = (u128)ua * (uint64_t)ux;
+  const u128 p1 = (u128)ua * (uint64_t)(ux >> 64);
+  if ((uint64_t)(p1 >> 64) != 0) return 0;
+  const u128 m = p0 + (p1 << 64);
+  if (m < p0) return 0;
+  const u128 lim = ((u128)1 << 127) - (neg_r ? 0 : 1);
+  if (m > lim) return 0;
+  return neg_r ? (i128)((u128)0 - m) : (i128)m;
+}
 Q32_FN int64_t det(const Mat& M) {
-  const i128 a = M.m[0], b = M.m[1], c = M.m[2], d = M.m[3], e = M.m[4], f = M.m[5], g = M.m[6],
-             h = M.m[7], i = M.m[8];
-  i128 r = ck_mul(a, ck_sub(ck_mul(e, i), ck_mul(f, h)));
-  r = ck_sub(r, ck_mul(b, ck_sub(ck_mul(d, i), ck_mul(f, g))));
-  r = ck_add(r, ck_mul(c, ck_sub(ck_mul(d, h), ck_mul(e, g))));
+  const int64_t a = M.m[0], b = M.m[1], c = M.m[2], d = M.m[3], e = M.m[4], f = M.m[5], g = M.m[6],
+                h = M.m[7], i = M.m[8];
+  i128 r = ck_mul_wide(a, ck_sub(wide_mul(e, i), wide_mul(f, h)));
+  r = ck_sub(r, ck_mul_wide(b, ck_sub(wide_mul(d, i), wide_mul(f, g))));
+  r = ck_add(r, ck_mul_wide(c, ck_sub(wide_mul(d, h), wide_mul(e, g))));
   // r / 2^32 / 2^32 truncating is r / 2^64 truncating
   const u128 m = r < 0 ? (u128)0 - (u128)r : (u128)r;
   const uint64_t q = (uint64_t)(m >> 64);
@@ -272,16 +334,17 @@ Q32_FN int64_t narrow(i128 x) {
   const uint64_t q = (uint64_t)(m >> 32);
   return (int64_t)(x < 0 ? 0 - q : q);
 }
-// adjugate / det, each cofactor narrowed then scaled by 1 / det (:730-762)
-Q32_FN Mat inverse(const Mat& M) {
-  const int64_t dt = det(M);
+// adjugate / det, each cofactor narrowed then scaled by 1 / det (:730-762);
+// dt = det(M)
+Q32_FN Mat inverse_with_det(const Mat& M, int64_t dt) {
   if (dt == 0) return mat_zero();
   const int64_t inv = div(ONE, dt);
-  const i128 a = M.m[0], b = M.m[1], c = M.m[2], d = M.m[3], e = M.m[4], f = M.m[5], g = M.m[6],
-             h = M.m[7], i = M.m[8];
-  // plain i128 arithmetic in the reference: wrapping here
-  auto w = [](i128 x, i128 y, i128 z, i128 t) -> i128 {
-    return (i128)((u128)x * (u128)y - (u128)z * (u128)t);
+  const int64_t a = M.m[0], b = M.m[1], c = M.m[2], d = M.m[3], e = M.m[4], f = M.m[5], g = M.m[6],
+                h = M.m[7], i = M.m[8];
+  // plain i128 arithmetic in the reference: the products fit, the
+  // difference wraps here
+  auto w = [](int64_t x, int64_t y, int64_t z, int64_t t) -> i128 {
+    return (i128)((u128)wide_mul(x, y) - (u128)wide_mul(z, t));
   };
   Mat o;
   o.m[0] = mul(narrow(w(e, i, f, h)), inv);
@@ -296,6 +359,8 @@ Q32_FN Mat inverse(const Mat& M) {
   return o;
 }
 
+Q32_FN Mat inverse(const Mat& M) { return inverse_with_det(M, det(M)); }
+
 // quadratic B-spline weight (mpm.rs:146-156)
 Q32_FN int64_t bspline(int64_t f) {
   const int64_t a = qabs(f);
@@ -309,7 +374,7 @@ Q32_FN int64_t bspline(int64_t f) {
 // (mu (F - F^-T) + lambda ln(Jc) F^-T) F^T / Jc
 Q32_FN Mat kirchhoff(const Mat& F, int64_t j, int64_t mu, int64_t lambda, int64_t j_lo,
                      int64_t j_hi) {
-  const Mat fit = transpose(inverse(F));
+  const Mat fit = transpose(inverse_with_det(F, j));  // j = det(F)
   const int64_t jc = clamp(j, j_lo, j_hi);
   const Mat pk = mat_add(mat_scale(mat_sub(F, fit), mu), mat_scale(fit, mul(lambda, ln(jc))));
   return mat_div(matmul(pk, transpose(F)), jc);

@@ -17,8 +17,9 @@ the context stream -- and give the per-kernel durations behind `roofline`
 they are kept out of the window that gives `value`).  The launch shape is
 autotuned once before the warm-up.  `extra` adds the profiled window's own
 throughput, the 16M-particle sparse-grid workload, the deterministic Q32.32
-mode, the batched-scenes speedup, the mixed-precision workload and the
-engine's LOD tier tick on a 3M-capacity resident buffer.
+mode, the batched-scenes speedup, the mixed-precision workload, the
+engine's LOD tier tick on a 3M-capacity resident buffer and the all-integer
+FixedPoint MpmSimulation on 1M particles.
 """
 from __future__ import annotations
 
@@ -286,6 +287,46 @@ def extra_tiers(W, aemh, ticks=60, with_cpu=True):
     return out
 
 
+def extra_fixed(W, aemh, n=1 << 20, res=64, steps=8, warmup=2, with_cpu=True):
+    """The all-integer FixedPoint MpmSimulation (ae_physics/src/mpm.rs) on a
+    fresh cloud: `warmup` steps, then `steps` in one aemh_fixed_step call
+    (one substep each), bracketed by the call's own synchronize; beside it
+    the oracle (serial C++ restatement of the reference's loops, one
+    thread) on the same scene for 2 steps."""
+    import numpy as np
+    sc = W.fixed_cloud(n, res)
+    zero = np.zeros_like(sc.pos)
+    sim = aemh.FixedSim(sc.config)
+    sim.add_particles(sc.pos, zero, sc.mass)
+    sim.step(warmup)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    el = time.perf_counter() - t0
+    st = sim.state()
+    sim.close()
+    out = {"workload": f"{sc.name}: {sc.n} particles uniform in the middle half of a {res}^3 grid, "
+                       f"cell 0.5, E 50, density 1, dt 1/60, elastoplastic, 1 substep, Q32.32 "
+                       f"integer arithmetic, bit-exact with the reference's serial loops",
+           "steps": steps, "warmup": warmup,
+           "particle_updates_per_s": sc.n * steps / el,
+           "ms_per_step": round(el / steps * 1e3, 3),
+           "scatter_bound_frac": round(st["max_scatter_bound"] / float(1 << 39), 8),
+           "checked_substeps": st["checked_substeps"], "serial_substeps": st["serial_substeps"]}
+    if with_cpu:
+        from oracle import fixedsim as X
+        o = X.Sim(sc.config)
+        o.add_particles(sc.pos, zero, sc.mass)
+        t0 = time.perf_counter()
+        o.step(2)
+        elc = time.perf_counter() - t0
+        o.close()
+        out["cpu_port"] = {"value": sc.n * 2 / elc, "unit": "particle-updates/s", "cores": 1,
+                           "kind": "port",
+                           "sample": "oracle/fixedsim.cpp (serial, the reference's own loop "
+                                     "structure) on the same scene, its first 2 steps"}
+    return out
+
+
 def tier_cpu_port(tw, aemh, lod, near, mid, far, dt):
     """The oracle's cost of one average tick of the same world on the host:
     classification (vectorized f32, stable sort for the cull) amortized over
@@ -344,13 +385,23 @@ def probe(W, aemh, which="c2"):
     out, so FETCH_SIZE / WRITE_SIZE of its dispatch give the KiB each counter
     reports per real KiB; the dispatches of the 5 steps after the autotuner
     (the launch shape the benchmark uses) follow, on the 1M-particle snow
-    workload (c2) or the 16M-particle three-box workload (c3)."""
+    workload (c2) or the 16M-particle three-box workload (c3); or 10 steps of
+    the 1M-particle FixedPoint cloud (fixed)."""
     import torch
     x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
     for _ in range(3):
         y.copy_(x)
     torch.cuda.synchronize()
+    if which == "fixed":
+        import numpy as np
+        sc = W.fixed_cloud()
+        sim = aemh.FixedSim(sc.config)
+        sim.add_particles(sc.pos, np.zeros_like(sc.pos), sc.mass)
+        sim.step(10)
+        sim.close()
+        print("probe done: fixed cloud, 10 steps", flush=True)
+        return
     wl = W.c3_three_boxes() if which == "c3" else W.c2_snow_dam()
     eng = aemh.Engine()
     W.load_engine(eng, wl)
@@ -369,12 +420,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--extras", default="q32,c3,c4,c5,tiers",
-                    help="comma list of the extra workloads to run (q32, c3, c4, c5, tiers)")
-    ap.add_argument("--probe", nargs="?", const="c2", choices=("c2", "c3"),
+    ap.add_argument("--extras", default="q32,c3,c4,c5,tiers,fixed",
+                    help="comma list of the extra workloads to run (q32, c3, c4, c5, tiers, "
+                         "fixed)")
+    ap.add_argument("--probe", nargs="?", const="c2", choices=("c2", "c3", "fixed"),
                     help="short run for hardware-counter collection: a 256 MiB device copy "
                          "(calibrates FETCH_SIZE / WRITE_SIZE) then 5 steps of the main workload "
-                         "(c2, the default) or of the 16M-particle workload (c3)")
+                         "(c2, the default) or of the 16M-particle workload (c3), or 10 steps of "
+                         "the FixedPoint cloud (fixed)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: every box here has one GPU and the MPM step does "
@@ -436,6 +489,9 @@ def main():
     if "tiers" in want:
         log("[bench] LOD tier tick at engine scale")
         extra["tiers"] = extra_tiers(W, aemh, with_cpu=not args.no_cpu)
+    if "fixed" in want:
+        log("[bench] all-integer FixedPoint MpmSimulation, 1M particles")
+        extra["fixed"] = extra_fixed(W, aemh, with_cpu=not args.no_cpu)
     line["extra"] = extra
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")

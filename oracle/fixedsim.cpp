@@ -720,6 +720,14 @@ uint64_t fx_sim_add_particle(void* h, const int64_t* pos, const int64_t* vel, in
   s->p.push_back(q);
   return q.parent_id;
 }
+// add_particle n times in order (default color)
+uint64_t fx_sim_add_particles(void* h, uint64_t n, const int64_t* pos3, const int64_t* vel3, const int64_t* mass) {
+  const float col[4] = {0.8f, 0.6f, 0.4f, 1.0f};
+  uint64_t first = ((Sim*)h)->next_id;
+  ((Sim*)h)->p.reserve(((Sim*)h)->p.size() + n);
+  for (uint64_t i = 0; i < n; i++) fx_sim_add_particle(h, pos3 + i * 3, vel3 + i * 3, mass[i], col);
+  return first;
+}
 int fx_sim_remove_particle(void* h, uint64_t id) {  // :233-240
   Sim* s = (Sim*)h;
   for (size_t i = 0; i < s->p.size(); i++)

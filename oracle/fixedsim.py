@@ -47,6 +47,9 @@ def lib():
         L.fx_sim_add_particle.restype = C.c_uint64
         L.fx_sim_add_particle.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int64,
                                           C.POINTER(C.c_float)]
+        L.fx_sim_add_particles.restype = C.c_uint64
+        L.fx_sim_add_particles.argtypes = [P, C.c_uint64, C.POINTER(C.c_int64),
+                                           C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.fx_sim_remove_particle.restype = C.c_int
         L.fx_sim_remove_particle.argtypes = [P, C.c_uint64]
         L.fx_sim_set_model.argtypes = [P, C.c_uint32, C.c_int64]
@@ -142,7 +145,10 @@ def small_config(**kw):
 
 
 class Sim:
-    def __init__(self, cfg: FxConfig):
+    def __init__(self, cfg):
+        self.h = None
+        if not isinstance(cfg, FxConfig):  # any structure with the MpmConfig layout
+            cfg = FxConfig.from_buffer_copy(bytes(cfg))
         self.h = lib().fx_sim_new(C.byref(cfg))
 
     def close(self):
@@ -164,6 +170,14 @@ class Sim:
         v = (C.c_int64 * 3)(*vel)
         col = (C.c_float * 4)(*color)
         return int(lib().fx_sim_add_particle(self.h, p, v, int(mass), col))
+
+    def add_particles(self, pos, vel, mass):
+        pos = np.ascontiguousarray(pos, np.int64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, np.int64).reshape(-1, 3)
+        mass = np.ascontiguousarray(mass, np.int64).reshape(-1)
+        P64 = C.POINTER(C.c_int64)
+        return int(lib().fx_sim_add_particles(self.h, len(mass), pos.ctypes.data_as(P64),
+                                              vel.ctypes.data_as(P64), mass.ctypes.data_as(P64)))
 
     def remove_particle(self, pid):
         return bool(lib().fx_sim_remove_particle(self.h, pid))

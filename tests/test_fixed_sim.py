@@ -401,6 +401,28 @@ def test_gpu_fixed_scatter_check_paths():
 
 
 @pytest.mark.gpu
+def test_gpu_fixed_cloud_reused_order():
+    """20,000 particles in random order (the benchmark cloud, scaled down)
+    for 10 steps: P2G sorts them by base cell once and reuses that order for
+    later substeps while particles cross cells, so equal cells end up in
+    several runs of a wave.  Every step bit-exact."""
+    import aemh
+    from aemh import workloads as W
+    sc = W.fixed_cloud(20000, 32)
+    print("fixed cloud 20k, reused P2G order", flush=True)
+    g, o = _gpu_pair(sc.config)
+    zero = np.zeros_like(sc.pos)
+    g.add_particles(sc.pos, zero, sc.mass)
+    o.add_particles(sc.pos, zero, sc.mass)
+    for k in range(10):
+        g.step(1)
+        o.step(1)
+        _compare(g, o, f"cloud step {k + 1}")
+    g.close()
+    assert isinstance(sc.config, aemh.FixedConfig)
+
+
+@pytest.mark.gpu
 def test_gpu_fixed_snow_long():
     """Snow for 12 steps: the reference's SVD returns the rotated A^T A as V,
     so F = U S (A^T A)^T grows each substep and the stresses reach ~2e9."""
