@@ -15,9 +15,22 @@
 //   reference calls (fracture offsets, granular friction) are glibc's, as
 //   Rust's std on Linux.
 //
-// Pinned by the reference's unit tests of MpmSimulation
-// (ae_physics/src/mpm.rs:957-1148) and FixedPoint (fixed_point.rs:889-962),
-// restated in tests/test_fixed_sim.py, and by exact closed forms there.
+// Pinned in tests/test_fixed_sim.py:
+//   FixedPoint ops: the reference's unit tests (fixed_point.rs:889-962);
+//     mul / div / sqrt / neg against Python integer arithmetic on random
+//     and extreme operands; the sin table and its interpolation; ln against
+//     the truncated artanh series and math.log near 1; exp against math.exp;
+//     from_f32 / to_f32 against numpy; det / inverse on diagonal, random and
+//     overflowing matrices.
+//   MpmSimulation: the reference's unit tests (mpm.rs:957-1148); one
+//     substep of free fall (P2G, grid force, G2P) in exact integers;
+//     plasticity, granular and fluid returns against their float closed
+//     forms; the SVD quirk (V = the rotated A^T A), U and S, the snow clamp
+//     and rebuilt F in exact integers on a quarter-turned diagonal F;
+//     brittle's config switch; fracture on F = 1.2 I (fragment count,
+//     ring offsets, masses, ids, connections, event, no second break).
+// The grid scatter bound it reports is the device's quantity, compared
+// exactly in the GPU tests.
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
@@ -864,6 +877,38 @@ uint64_t fx_sim_snapshots(void* h, float* out13, uint8_t* fractured) {
   return k;
 }
 uint64_t fx_sim_scatter_bound(void* h) { return ((Sim*)h)->max_bound; }
+// handle_fracture alone (mpm.rs:641-716), for its closed-form tests
+void fx_sim_fracture_only(void* h) { ((Sim*)h)->fracture(); }
+// handle_constitutive_model (mpm.rs:456-472) alone on one particle of the
+// given F, stress (column-major), velocity and plastic strain, under the
+// config's model; out: F9, stress9, velocity3, plastic, and the config
+// fields the model rewrites (hardening, fracture_strain, enable_fracture)
+void fx_constitutive(const fx_config* c, const int64_t* F9, const int64_t* S9, const int64_t* vel3,
+                     int64_t plastic, int64_t* out25) {
+  fx_config cc = *c;
+  cc.particle_count = 0;
+  Sim* s = (Sim*)fx_sim_new(&cc);
+  Particle q;
+  for (int col = 0; col < 3; col++)
+    for (int r = 0; r < 3; r++) {
+      el(q.F.c[col], r) = R(F9[col * 3 + r]);
+      el(q.stress.c[col], r) = R(S9[col * 3 + r]);
+    }
+  q.vel = v3(R(vel3[0]), R(vel3[1]), R(vel3[2]));
+  q.plastic = R(plastic);
+  s->p.push_back(q);
+  s->constitutive();
+  const Particle& o = s->p[0];
+  for (int col = 0; col < 3; col++)
+    for (int r = 0; r < 3; r++) {
+      out25[col * 3 + r] = elc(o.F.c[col], r).raw;
+      out25[9 + col * 3 + r] = elc(o.stress.c[col], r).raw;
+    }
+  out25[18] = o.vel.x.raw, out25[19] = o.vel.y.raw, out25[20] = o.vel.z.raw;
+  out25[21] = o.plastic.raw;
+  out25[22] = s->hardening.raw, out25[23] = s->fracture_strain.raw, out25[24] = s->enable_fracture;
+  delete s;
+}
 // Scalar FixedPoint operations for the unit tests.
 int64_t fx_op(int op, int64_t a, int64_t b) {
   build_sin_table();

@@ -68,6 +68,9 @@ def lib():
                                    C.POINTER(C.c_float)]
         L.fx_sim_scatter_bound.restype = C.c_uint64
         L.fx_sim_scatter_bound.argtypes = [P]
+        L.fx_sim_fracture_only.argtypes = [P]
+        L.fx_constitutive.argtypes = [C.POINTER(FxConfig), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int64)]
         L.fx_sim_stats.argtypes = [P, C.POINTER(C.c_int64)]
         L.fx_sim_snapshots.restype = C.c_uint64
         L.fx_sim_snapshots.argtypes = [P, C.POINTER(C.c_float), C.POINTER(C.c_uint8)]
@@ -106,6 +109,19 @@ def mat_op(which, m9):
     lib().fx_mat_op(which, m.ctypes.data_as(C.POINTER(C.c_int64)),
                     out.ctypes.data_as(C.POINTER(C.c_int64)))
     return out
+
+
+def constitutive(cfg, F9, S9, vel3=(0, 0, 0), plastic=0):
+    """handle_constitutive_model on one particle (F, stress column-major);
+    returns (F9, S9, vel3, plastic, hardening, fracture_strain, fracture_on)."""
+    P64 = C.POINTER(C.c_int64)
+    f = np.ascontiguousarray(F9, np.int64)
+    st = np.ascontiguousarray(S9, np.int64)
+    v = np.ascontiguousarray(vel3, np.int64)
+    out = np.zeros(25, np.int64)
+    lib().fx_constitutive(C.byref(cfg), f.ctypes.data_as(P64), st.ctypes.data_as(P64),
+                          v.ctypes.data_as(P64), int(plastic), out.ctypes.data_as(P64))
+    return out[:9], out[9:18], out[18:21], int(out[21]), int(out[22]), int(out[23]), int(out[24])
 
 
 def default_config(**kw):
@@ -251,6 +267,9 @@ class Sim:
         k = lib().fx_sim_snapshots(self.h, f.ctypes.data_as(C.POINTER(C.c_float)),
                                    fr.ctypes.data_as(C.POINTER(C.c_uint8)))
         return f[:k], fr[:k]
+
+    def fracture_only(self):
+        lib().fx_sim_fracture_only(self.h)
 
     def scatter_bound(self):
         """Largest per-substep sum of (|c| >> 24) + 1 over the nonzero grid

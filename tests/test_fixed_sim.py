@@ -243,6 +243,173 @@ def test_sim_free_fall_one_substep_closed_form():
 
 
 # ---------------------------------------------------------------------------
+# Material models and fracture: invariants and closed forms on one particle
+# ---------------------------------------------------------------------------
+def qmul(a, b):
+    return wrap64(trunc_div(a * b, 1 << 32))
+
+
+def qdiv(a, b):
+    return wrap64(trunc_div(a * (1 << 32), b))
+
+
+def cols(m):
+    """3x3 nested list (rows) of raw values -> column-major 9 raw values."""
+    return [m[r][c] for c in range(3) for r in range(3)]
+
+
+def rows_f(v9):
+    """column-major raw 9 -> float rows."""
+    return np.array(v9, np.float64).reshape(3, 3).T / 2 ** 32
+
+
+def _stress(diag, shear):
+    q = [[X.from_f32(diag[0]), X.from_f32(shear), 0], [X.from_f32(shear), X.from_f32(diag[1]), 0],
+         [0, 0, X.from_f32(diag[2])]]
+    return cols(q)
+
+
+IDENT = cols([[ONE, 0, 0], [0, ONE, 0], [0, 0, ONE]])
+
+
+def test_plasticity_return_mapping():
+    """handle_plasticity (mpm.rs:474-502): above the limit yield + hardening
+    * plastic the deviator is scaled by limit / vm with vm = sqrt(1.5 |dev|)
+    (the reference's measure), the trace kept and plastic grows by
+    (vm - limit) / E; below it nothing changes."""
+    cfg = X.default_config(model=X.ELASTOPLASTIC, yield_=X.from_f32(1.0))
+    S = _stress((3.0, -1.0, 0.5), 2.0)
+    F, S2, _, pl, *_ = X.constitutive(cfg, IDENT, S, plastic=X.from_f32(2.0))
+    s, s2 = rows_f(S), rows_f(S2)
+    h = np.trace(s) / 3
+    dev = s - h * np.eye(3)
+    dn = math.sqrt((np.diag(dev) ** 2).sum() + 2 * (dev[0, 1] ** 2 + dev[0, 2] ** 2 + dev[1, 2] ** 2))
+    vm = math.sqrt(1.5 * dn)
+    limit = 1.0 + 0.1 * 2.0
+    assert vm > limit
+    assert abs(np.trace(s2) - np.trace(s)) < 1e-8
+    assert np.abs((s2 - h * np.eye(3)) - dev * (limit / vm)).max() < 1e-7
+    assert abs((pl - X.from_f32(2.0)) / 2 ** 32 - (vm - limit) / 1e6) < 1e-9
+    assert list(F) == IDENT
+    small = _stress((0.1, 0.0, 0.0), 0.0)
+    _, S3, _, pl3, *_ = X.constitutive(cfg, IDENT, small)
+    assert list(S3) == small and pl3 == 0
+
+
+def test_granular_cap_and_tension():
+    """handle_granular (mpm.rs:504-540): tension clears the stress; in
+    compression the deviator norm is capped at tan(phi) p + yield."""
+    cfg = X.default_config(model=X.GRANULAR, model_param=X.from_f32(30.0), yield_=X.from_f32(1.0))
+    _, S, _, pl, *_ = X.constitutive(cfg, IDENT, _stress((1.0, 1.0, 1.0), 0.0))
+    assert not any(S) and pl == 0
+    Sin = _stress((-3.0, -1.0, -2.0), 4.0)
+    _, S2, _, pl2, *_ = X.constitutive(cfg, IDENT, Sin)
+    s, s2 = rows_f(Sin), rows_f(S2)
+    h = np.trace(s) / 3
+    dev = s - h * np.eye(3)
+    dn = math.sqrt((np.diag(dev) ** 2).sum() + 2 * (dev[0, 1] ** 2 + dev[0, 2] ** 2 + dev[1, 2] ** 2))
+    cap = math.tan(math.radians(30.0)) * (-h) + 1.0
+    assert dn > cap
+    assert np.abs((s2 - h * np.eye(3)) - dev * (cap / dn)).max() < 1e-6
+    assert abs(pl2 / 2 ** 32 - (dn - cap) / 1e6) < 1e-9
+
+
+def test_fluid_pressure_and_damping():
+    """handle_fluid (mpm.rs:542-562): stress -p I with p = -tr / 3 (zero in
+    tension); velocity times exp(-viscosity |v|), exp being the ten-term
+    Taylor sum of FixedPoint::exp."""
+    cfg = X.default_config(model=X.FLUID, model_param=X.from_f32(0.5))
+    Sin = _stress((-3.0, -1.0, -2.0), 4.0)
+    v = [X.from_f32(3.0), X.from_f32(4.0), 0]
+    _, S, v2, *_ = X.constitutive(cfg, IDENT, Sin, vel3=v)
+    tr = Sin[0] + Sin[4] + Sin[8]
+    p = trunc_div(-tr, 3)
+    assert list(S) == cols([[-p, 0, 0], [0, -p, 0], [0, 0, -p]])
+    x = -0.5 * 5.0
+    taylor = sum(x ** k / math.factorial(k) for k in range(12))
+    assert np.allclose(np.array(v2[:2]) / 2 ** 32, np.array([3.0, 4.0]) * taylor, rtol=1e-6, atol=1e-8)
+    _, S3, *_ = X.constitutive(cfg, IDENT, _stress((1.0, 2.0, 3.0), 0.0))
+    assert not any(S3)
+
+
+def test_svd_quirk_and_snow_clamp():
+    """fixed_mat3_svd (mpm.rs:794-888) on A = R D (R a quarter turn about z):
+    A^T A is already diagonal, so no Jacobi sweep runs and the closed form
+    holds: S = sqrt(diag(A^T A)), V = A^T A itself (the reference hands back
+    the rotated matrix, not the rotations), U = A V / S.  Snow
+    (mpm.rs:565-639) clamps S to [1 - 2.5e-2, 1 + 7.5e-3] and rebuilds
+    F = U S V^T; hardening takes the model's value."""
+    d = [X.from_f32(1.1), X.from_f32(0.9), ONE]
+    A = [[0, -d[1], 0], [d[0], 0, 0], [0, 0, d[2]]]
+    u, sv, v = X.mat_op(2, np.array(cols(A), np.int64)).reshape(3, 9)
+    g = [qmul(di, di) for di in d]
+    s = [math.isqrt(gi << 32) for gi in g]
+    assert list(sv[[0, 4, 8]]) == s
+    assert list(v) == cols([[g[0], 0, 0], [0, g[1], 0], [0, 0, g[2]]])
+    U = [[qmul(qmul(A[i][j], g[j]), qdiv(ONE, s[j])) for j in range(3)] for i in range(3)]
+    assert list(u) == cols(U)
+    cfg = X.default_config(model=X.SNOW, model_param=X.from_f32(0.05))
+    F2, _, _, _, hard, *_ = X.constitutive(cfg, cols(A), [0] * 9)
+    lo, hi = ONE - X.from_f32(2.5e-2), ONE + X.from_f32(7.5e-3)
+    sc = [min(max(si, lo), hi) if not (lo <= si <= hi) else si for si in s]
+
+    def mm(P, Q):
+        return [[wrap64(sum(qmul(P[r][k], Q[k][c]) for k in range(3))) for c in range(3)] for r in range(3)]
+    US = mm(U, [[sc[0], 0, 0], [0, sc[1], 0], [0, 0, sc[2]]])
+    Vt = [[g[0], 0, 0], [0, g[1], 0], [0, 0, g[2]]]
+    assert list(F2) == cols(mm(US, Vt))
+    assert hard == X.from_f32(0.05)
+    # brittle: plasticity, then the model's threshold switches fracture on
+    cfg = X.default_config(model=X.BRITTLE, model_param=X.from_f32(0.02))
+    *_, fs, on = X.constitutive(cfg, IDENT, [0] * 9)
+    assert fs == X.from_f32(0.02) and on == 1
+
+
+def test_fracture_closed_form():
+    """handle_fracture (mpm.rs:641-716) on one particle with F = 1.2 I:
+    strain = tr(F^T F) / 3 - 1 = 0.44 against 0.15 gives
+    min(0.44 / 0.15, 8) -> 2 + 1 = 3 fragments on a ring of half a cell (y
+    flattened to 0.2 cell, z from sin(1.7 i)), each with mass / 4, ids and
+    connections in order, velocity + 2 offset, and one event."""
+    cfg = X.small_config(particle_count=0)
+    o = X.Sim(cfg)
+    pos = [X.from_f32(1.0), X.from_f32(1.0), X.from_f32(1.0)]
+    pid = o.add_particle(pos, (X.from_f32(0.5), 0, 0), X.from_f32(0.2))
+    d = o.read()
+    q = d["q"].copy()
+    f = X.from_f32(1.2)
+    q[0, 6:15] = cols([[f, 0, 0], [0, f, 0], [0, 0, f]])
+    o.write(0, q, d["flags"], d["ids"], d["color"])
+    thr = X.from_f32(0.15)
+    o.set_fracture(True, thr)
+    o.fracture_only()
+    strain = trunc_div(3 * qmul(f, f), 3) - ONE
+    nf = int(min(np.float32(qdiv(strain, thr)) / np.float32(2 ** 32), np.float32(8.0))) + 1
+    assert nf == 3
+    d = o.read()
+    assert o.count() == 1 + nf and d["flags"][0].tolist() == [1, 1, 0]
+    assert d["q"][0, 24] == X.from_f32(0.2) // (nf + 1)
+    cell = 0.5
+    for k in range(nf):
+        r = 1 + k
+        ang = np.float32(np.float32(k) * np.float32(2 * math.pi)) / np.float32(nf)
+        off = np.array([math.cos(ang) * cell / 2, math.sin(ang) * cell * 0.2,
+                        math.sin(float(np.float32(np.float32(k) * np.float32(1.7)))) * cell / 2])
+        assert np.abs(d["q"][r, 0:3] / 2 ** 32 - (1.0 + off)).max() < 1e-6
+        assert np.abs(d["q"][r, 3:6] / 2 ** 32 - (np.array([0.5, 0, 0]) + 2 * off)).max() < 1e-6
+        assert d["q"][r, 24] == X.from_f32(0.2) // (nf + 1)
+        assert d["ids"][r, 0] == pid + 1 + k
+        assert d["ids"][r, 1:].tolist() == [pid + 1 + c for c in range(min(nf, 7))] + [0] * (8 - min(nf, 7))
+        assert d["flags"][r].tolist() == [1, 0, min(nf, 8)]
+        assert d["q"][r, 6:15].tolist() == IDENT
+    ev = o.events()
+    assert ev["frags"].tolist() == [nf] and ev["strain"][0] == np.float32(strain / 2 ** 32)
+    assert ev["pos"][0].tolist() == [1.0, 1.0, 1.0]
+    o.fracture_only()  # a fractured parent does not break again; fragments have F = I
+    assert o.count() == 1 + nf
+
+
+# ---------------------------------------------------------------------------
 # GPU against the oracle
 # ---------------------------------------------------------------------------
 BOUND_LIMIT = 1 << 39  # sum of (|c| >> 24) + 1 below this: the atomic scatter is exact
