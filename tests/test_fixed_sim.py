@@ -590,6 +590,23 @@ def test_gpu_fixed_cloud_reused_order():
 
 
 @pytest.mark.gpu
+def test_gpu_fixed_brittle_cascade_growth():
+    """Brittle fragments break again: 125 particles become 9,588 with 4,043
+    events in 10 steps, growing the particle store, the event array and the
+    P2G order buffers past their first sizes; every step bit-exact."""
+    cfg = X.small_config(model=X.BRITTLE, model_param=X.from_f32(0.02), particle_count=125, substeps=2)
+    print("fixed brittle cascade", flush=True)
+    g, o = _gpu_pair(cfg)
+    for k in range(10):
+        g.step(1)
+        o.step(1)
+        _compare(g, o, f"brittle step {k + 1}")
+    assert o.count() > 4096 and o.state()["n_events"] > 1024
+    _compare_snapshots(g, o)
+    g.close()
+
+
+@pytest.mark.gpu
 def test_gpu_fixed_snow_long():
     """Snow for 12 steps: the reference's SVD returns the rotated A^T A as V,
     so F = U S (A^T A)^T grows each substep and the stresses reach ~2e9."""
