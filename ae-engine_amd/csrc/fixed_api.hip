@@ -543,14 +543,18 @@ aemh_status aemh_fixed_remove_particle(aemh_fixed* s, uint64_t id, int* removed)
   FXTRY(s, hipMemcpyAsync(&idx, s->find_idx, 4, hipMemcpyDeviceToHost, s->stream));
   FXTRY(s, hipStreamSynchronize(s->stream));
   if (idx == UINT32_MAX) return AEMH_OK;
-  // Vec::remove: the tail moves down one slot, order kept
+  // Vec::remove: the tail moves down one slot, order kept, in chunks from
+  // the front (each chunk reads the slots just past those it writes)
   const uint32_t tail = s->n - idx - 1;
-  if (tail) {
-    FXCHK(ensure_stage(s, (size_t)tail * FX_NPLANES));
-    FXTRY(s, fx_launch_get(s->st, idx + 1, tail, s->stage, s->stream));
-    FXTRY(s, fx_launch_put(s->st, idx, s->stage, tail, s->stream));
-    FXTRY(s, hipStreamSynchronize(s->stream));
+  const uint32_t chunk = 1u << 20;
+  for (uint32_t done = 0; done < tail;) {
+    const uint32_t m = std::min(chunk, tail - done);
+    FXCHK(ensure_stage(s, (size_t)m * FX_NPLANES));
+    FXTRY(s, fx_launch_get(s->st, idx + 1 + done, m, s->stage, s->stream));
+    FXTRY(s, fx_launch_put(s->st, idx + done, s->stage, m, s->stream));
+    done += m;
   }
+  FXTRY(s, hipStreamSynchronize(s->stream));
   s->n -= 1;
   s->order_valid = false;
   if (removed) *removed = 1;

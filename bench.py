@@ -492,6 +492,8 @@ def main():
     if "fixed" in want:
         log("[bench] all-integer FixedPoint MpmSimulation, 1M particles")
         extra["fixed"] = extra_fixed(W, aemh, with_cpu=not args.no_cpu)
+        log("[bench] all-integer FixedPoint MpmSimulation, 16M particles (same density, 160^3)")
+        extra["fixed_16m"] = extra_fixed(W, aemh, n=1 << 24, res=160, with_cpu=False)
     line["extra"] = extra
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")

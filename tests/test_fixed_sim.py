@@ -607,6 +607,74 @@ def test_gpu_fixed_brittle_cascade_growth():
 
 
 @pytest.mark.gpu
+def test_gpu_fixed_random_api_sequence():
+    """A seeded random sequence of the API between steps -- add one or many
+    particles, remove (first, middle, last, missing), write records (move,
+    push, deactivate, reactivate), switch models, toggle fracture, step one
+    or several times -- on a 2,000-particle cloud.  The P2G order the device
+    keeps across substeps must follow every change of the particle set;
+    every field is compared after every operation."""
+    from aemh import workloads as W
+    rng = np.random.default_rng(2024)
+    sc = W.fixed_cloud(2000, 32)
+    g, o = _gpu_pair(sc.config)
+    zero = np.zeros_like(sc.pos)
+    g.add_particles(sc.pos, zero, sc.mass)
+    o.add_particles(sc.pos, zero, sc.mass)
+    models = [(X.ELASTOPLASTIC, 0), (X.ELASTIC, 0), (X.FLUID, X.from_f32(0.01)),
+              (X.GRANULAR, X.from_f32(30.0)), (X.SNOW, X.from_f32(0.05))]
+    for it in range(40):
+        op = rng.integers(0, 7)
+        n = o.count()
+        if op == 0:  # add one
+            p = [X.from_f32(v) for v in rng.uniform(4.0, 12.0, 3)]
+            v = [X.from_f32(v) for v in rng.uniform(-1.0, 1.0, 3)]
+            m = X.from_f32(float(rng.uniform(0.5, 2.0)))
+            assert g.add_particle(p, v, m) == o.add_particle(p, v, m)
+        elif op == 1:  # add many
+            k = int(rng.integers(1, 300))
+            p = np.array([[X.from_f32(v) for v in q] for q in rng.uniform(4.0, 12.0, (k, 3))], np.int64)
+            m = np.full(k, X.from_f32(1.5), np.int64)
+            col = rng.uniform(0, 1, (k, 4)).astype(np.float32)
+            g.add_particles(p, np.zeros_like(p), m, col)
+            for i in range(k):
+                o.add_particle(p[i], (0, 0, 0), m[i], tuple(col[i]))
+        elif op == 2 and n:  # remove
+            ids = o.read()["ids"][:, 0]
+            pick = [ids[0], ids[n // 2], ids[-1], 10 ** 12][int(rng.integers(0, 4))]
+            assert g.remove_particle(int(pick)) == o.remove_particle(int(pick))
+        elif op == 3 and n:  # write a slice of records
+            first = int(rng.integers(0, n))
+            cnt = int(rng.integers(1, min(200, n - first) + 1))
+            d = o.read()
+            q, fl = d["q"][first:first + cnt].copy(), d["flags"][first:first + cnt].copy()
+            q[:, 0:3] += rng.integers(-(1 << 31), 1 << 31, (cnt, 3))
+            q[:, 3:6] = rng.integers(-(1 << 32), 1 << 32, (cnt, 3))
+            fl[:, 0] = rng.integers(0, 2, cnt)
+            for s_ in (g, o):
+                s_.write(first, q, fl, d["ids"][first:first + cnt], d["color"][first:first + cnt])
+        elif op == 4:  # switch model
+            m, prm = models[int(rng.integers(0, len(models)))]
+            g.set_model(m, prm)
+            o.set_model(m, prm)
+        elif op == 5:  # one step with fracture on (fragments of this exploding
+            # scene would break again and again if it stayed on)
+            thr = X.from_f32(float(rng.uniform(0.05, 0.5)))
+            for s_ in (g, o):
+                s_.set_fracture(True, thr)
+                s_.step(1)
+                s_.set_fracture(False)
+            _compare(g, o, f"op {it} (fracture step)")
+        k = int(rng.integers(1, 4))
+        g.step(k)
+        o.step(k)
+        print(f"random api op {it}: kind {op}, {o.count()} particles", flush=True)
+        _compare(g, o, f"op {it} ({op})")
+    _compare_snapshots(g, o)
+    g.close()
+
+
+@pytest.mark.gpu
 def test_gpu_fixed_snow_long():
     """Snow for 12 steps: the reference's SVD returns the rotated A^T A as V,
     so F = U S (A^T A)^T grows each substep and the stresses reach ~2e9."""
