@@ -47,7 +47,6 @@ struct aemh_fixed {
   uint32_t* find_idx = nullptr;
   FxSortBufs sort{};
   uint32_t sort_cap = 0;
-  bool sorted_p2g = true;
   uint32_t since_sort = 0;    // substeps since the P2G order was sorted
   bool order_valid = false;   // perm is a permutation of the current particles
   double time = 0.0;
@@ -62,6 +61,9 @@ constexpr uint32_t kMaxParticles = 1u << 30;
 // P2G reuses its cell order this many substeps (particles move under a
 // cell per substep, so the runs of equal cells stay mostly whole)
 constexpr uint32_t kFxResortEvery = 4;
+// Below this many particles P2G scatters in particle order with one atomic
+// per value: the cell sort's extra launches cost more than they save.
+constexpr uint32_t kFxSortMin = 8192;
 
 aemh_status fxfail(aemh_fixed* s, aemh_status code, const char* fmt, ...) {
   char buf[512];
@@ -373,18 +375,20 @@ aemh_status substep(aemh_fixed* s, int64_t sdt) {
   sp.n = s->n;
   FXTRY(s, hipMemsetAsync(s->grid, 0, (size_t)s->ncells * 32, s->stream));
   int resort = FX_ORDER_KEEP;
-  if (s->sorted_p2g) {
+  const bool sorted = s->n >= kFxSortMin;
+  if (sorted) {
     FXCHK(ensure_sort(s, s->n));
     resort = !s->order_valid ? FX_ORDER_FROM_PLANES
                              : (s->since_sort >= kFxResortEvery ? FX_ORDER_RESORT : FX_ORDER_KEEP);
     s->since_sort = resort != FX_ORDER_KEEP ? 1 : s->since_sort + 1;
     s->order_valid = true;
+  } else {
+    s->order_valid = false;  // G2P does not refresh the records on this path
   }
   FXTRY(s, fx_launch_p2g(s->st, sp, s->grid, s->bound, s->cell_bound, s->ncells, s->counters,
-                         s->sorted_p2g ? &s->sort : nullptr, resort, s->stream));
+                         sorted ? &s->sort : nullptr, resort, s->stream));
   FXTRY(s, fx_launch_grid(s->grid, s->ncells, sp.gdt, s->bound, s->stream));
-  FXTRY(s, fx_launch_g2p((int)model, s->st, sp, s->grid, s->sin_tab, s->sorted_p2g ? &s->sort : nullptr,
-                         s->stream));
+  FXTRY(s, fx_launch_g2p((int)model, s->st, sp, s->grid, s->sin_tab, sorted ? &s->sort : nullptr, s->stream));
   if (model == AEMH_FX_BRITTLE) {  // :462-465
     c.fracture_strain = c.model_param;
     c.enable_fracture = 1;
