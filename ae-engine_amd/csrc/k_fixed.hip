@@ -11,6 +11,10 @@
 //     pass bounds each cell, and if a cell can reach the limit the grid is
 //     rebuilt by the serial loop on the device.  The result is the
 //     reference's in every case; the check costs two near-empty launches.
+//   * Because the sums are order-free, large scenes scatter in an order of
+//     their own: particles counting-sorted by base cell, read as packed
+//     records that G2P refreshes, with the contributions of a wave's run of
+//     equal cells summed by shuffles so one lane per run issues the atomic.
 //   * grid forces and grid velocity (momentum / mass) are per cell; the
 //     division is done once per cell instead of once per particle-node pair
 //     in G2P (the same operands give the same quotient).
@@ -167,6 +171,7 @@ __device__ inline uint32_t agg_key(const PAt& P, const FxSubstep& sp, uint32_t n
 // any order, which the exact integer scatter does not see.  The source is
 // either the particle planes in particle order (after the particle set
 // changed), keys computed here, or the previous order's records and keys.
+
 // Runs of equal keys within a wave (a sorted source has long ones): the
 // run's first lane and each lane's position in its run.  Every lane calls
 // it (lanes past the end pass a key of their own that matches no other).
