@@ -114,6 +114,13 @@ __device__ __forceinline__ long long rint_i64(double y) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(y + 6755399441055744.0);
   return (long long)(b - 0x4338000000000000ull);
 }
+// rint_i64(v * scale) for a power-of-two scale with |v scale| < 2^51, in
+// three VALU: the product is exact, so one fused multiply-add with the same
+// constant rounds exactly as the separate product and sum do.
+__device__ __forceinline__ long long rint_i64_scaled(float v, double scale) {
+  const double y = __builtin_fma((double)v, scale, 6755399441055744.0);
+  return (long long)((unsigned long long)__double_as_longlong(y) - 0x4338000000000000ull);
+}
 
 #if AEMH_DET
 // FixedPoint::from_f32: trunc(v 2^32) with saturation, NaN -> 0.  The product
@@ -339,12 +346,12 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
 
 #if !AEMH_DET
   int eg[3];
-  float scale[3];
+  double scale[3];
 #pragma unroll
   for (int g = 0; g < 3; g++) {
     eg[g] = max(max(wexp[0][g], wexp[1][g]), max(wexp[2][g], wexp[3][g]));
     eg[g] = max(eg[g], -70);
-    scale[g] = __int_as_float((127 + 50 - eg[g]) << 23);  // 2^(50 - e_g)
+    scale[g] = (double)__int_as_float((127 + 50 - eg[g]) << 23);  // 2^(50 - e_g)
   }
 #endif
 
@@ -376,7 +383,7 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
 #else
       auto put = [&](int f, int di, int dj, int dk, int, int, int, float v) {
         const int g = f == 0 ? 0 : (f < 4 ? 1 : 2);
-        const long long a = rint_i64((double)(v * scale[g]));  // |v scale| < 2^50
+        const long long a = rint_i64_scaled(v, scale[g]);  // |v scale| < 2^50
         atomicAdd((unsigned long long*)&tile[tb + f * kTileNodes + (dk * kTileSide + dj) * kTileSide + di],
                   (unsigned long long)a);
       };
