@@ -10,7 +10,9 @@ GPU: aemh_fixed_* through the C ABI against the oracle, every field of every
 particle bit-exact, for each material model, substeps, added and removed
 particles and fracture.
 """
+import glob
 import math
+import os
 
 import numpy as np
 import pytest
@@ -407,6 +409,48 @@ def test_fracture_closed_form():
     assert ev["pos"][0].tolist() == [1.0, 1.0, 1.0]
     o.fracture_only()  # a fractured parent does not break again; fragments have F = I
     assert o.count() == 1 + nf
+
+
+# ---------------------------------------------------------------------------
+# Golden fixtures (tests/golden/fixed_*.npz, made by make_golden_fixed.py)
+# ---------------------------------------------------------------------------
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "fixed_*.npz")))
+
+
+def _golden_config(z):
+    return X.FxConfig.from_buffer_copy(z["config"].tobytes())
+
+
+def _check_golden(sim, z, where):
+    d = sim.read()
+    for k in ("q", "flags", "ids", "color"):
+        assert np.array_equal(d[k], z[k]), (where, k)
+    st = sim.state()
+    assert st["step_count"] == int(z["step_count"]) and st["next_id"] == int(z["next_id"]), where
+    assert st["time"] == float(z["time"]), where
+    ev = sim.events()
+    for k, zk in (("time", "ev_time"), ("pos", "ev_pos"), ("strain", "ev_strain"), ("frags", "ev_frags")):
+        assert np.array_equal(ev[k], z[zk]), (where, k)
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_oracle_matches_golden_fixture(path):
+    z = np.load(path)
+    s = X.Sim(_golden_config(z))
+    s.step(int(z["steps"]))
+    _check_golden(s, z, os.path.basename(path))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_gpu_fixed_matches_golden_fixture(path):
+    import aemh
+    z = np.load(path)
+    g = aemh.FixedSim(_golden_config(z))
+    g.step(int(z["steps"]))
+    _check_golden(g, z, os.path.basename(path))
+    g.close()
 
 
 # ---------------------------------------------------------------------------
