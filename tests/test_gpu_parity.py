@@ -128,6 +128,28 @@ def test_snow_scaled_dam_deterministic_bitexact():
     assert np.any(d["Jp"] != 1.0)  # the snow clamp was exercised
 
 
+def test_three_boxes_scaled_deterministic_bitexact():
+    """The 16M three-material workload (fluid, jelly, snow falling onto the
+    floor) at its particle density, scaled: 196,608 particles on 64^3, 6
+    steps; the step picks its launch shape with the autotuner first, as the
+    benchmark does."""
+    wl = W.c3_three_boxes(n=3 * 65536, steps=6, res=64)
+    wl.accum = aemh.ACCUM_Q32_32
+    eng = gpu_engine(wl)
+    eng.autotune()
+    cfg = oracle_cfg(wl)
+    cfg_threads = R.make_config(*wl.grid[:4], wl.grid[4], 1, wl.gravity, 0.995, wl.bc_margin, 0.0,
+                                R.ACCUM_Q32, R.CLAMP_AUTO, threads=8)
+    st = oracle_state(wl)
+    M = oracle_mats(wl.materials)
+    for k in range(6):
+        eng.step(wl.dt)
+        # Q32.32 sums do not depend on order: the threaded oracle is exact too
+        R.step(cfg_threads if k % 2 else cfg, M, st, wl.dt)
+    d = eng.download()
+    assert_bitexact(d, st)
+
+
 def test_velocity_only_deterministic_bitexact():
     wl = W.c1_elastic_cube(n=20000)
     wl.accum = aemh.ACCUM_Q32_32
