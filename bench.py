@@ -430,9 +430,8 @@ def main():
                          "the FixedPoint cloud (fixed)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one MI355X: every box here has one GPU and the MPM step does "
-              "not shard across GPUs in this build (see DESIGN.md for how it would).",
-              file=sys.stderr)
+        print("bench.py measures one MI355X: the MPM step runs on a single GPU and does not "
+              "shard across GPUs; run it with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     import aemh
     from aemh import workloads as W
