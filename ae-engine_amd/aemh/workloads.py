@@ -152,16 +152,18 @@ def c4_batched_scenes(n_scenes=256, steps=100, n_min=4096, n_max=262144):
     return wl
 
 
-def c5_mixed_precision(n=1 << 22, steps=500, res=192, storage=STORE_F32):
+def c5_mixed_precision(n=1 << 22, steps=500, res=192, storage=STORE_F32, nh_only=False):
     """4,194,304 particles on a 192^3 grid: a snow box and a Neo-Hookean box
-    (E 1e4, nu 0.2, rho 10) side by side; dt 5e-5."""
+    (E 1e4, nu 0.2, rho 10) side by side; dt 5e-5.  nh_only makes both boxes
+    Neo-Hookean (no snow SVD in G2P, which then is bound by HBM traffic)."""
     half = n // 2
     p0 = uniform_box(SEED_BASE + 4, half, (0.15, 0.2, 0.2), (0.45, 0.5, 0.8))
     p1 = uniform_box(SEED_BASE + 4 + 1000, n - half, (0.55, 0.2, 0.2), (0.85, 0.5, 0.8))
     vol = 0.3 * 0.3 * 0.6
-    mass = np.concatenate([np.full(half, 400.0 * vol / half, np.float32),
+    rho0 = 10.0 if nh_only else 400.0
+    mass = np.concatenate([np.full(half, rho0 * vol / half, np.float32),
                            np.full(n - half, 10.0 * vol / (n - half), np.float32)])
-    mat = np.concatenate([np.zeros(half, np.uint16), np.ones(n - half, np.uint16)])
+    mat = np.concatenate([np.full(half, 1 if nh_only else 0, np.uint16), np.ones(n - half, np.uint16)])
     mats = [snow_material(), Material(young_modulus=1e4, poisson_ratio=0.2, density=10.0,
                                       model=NEO_HOOKEAN)]
     return Workload("c5_mixed_4m_192", np.concatenate([p0, p1]), mass, mat, mats,
