@@ -204,8 +204,7 @@ void carve_store(char* p, PBuf& pb, uint32_t n, int storage) {
       pb.C[k] = carve<float>(p, n);
       pb.F[k] = carve<float>(p, n);
     } else {
-      pb.Ch[k] = carve<uint16_t>(p, n);
-      pb.Fh[k] = carve<uint16_t>(p, n);
+      pb.CF[k] = carve<uint32_t>(p, n);
     }
   }
   pb.J = carve<float>(p, n);

@@ -60,8 +60,10 @@ struct PBuf {
   float* v[3];
   float* C[9];
   float* F[9];
-  uint16_t* Ch[9];  // 16-bit storage of C (mixed precision)
-  uint16_t* Fh[9];  // 16-bit storage of F - I (mixed precision)
+  // 16-bit storage (mixed precision): two 16-bit values per 32-bit word.
+  // Words 0-3 hold C pairs (C[2w] in the low half, C[2w+1] in the high),
+  // words 4-7 the pairs of F - I, word 8 C[8] (low) and F[8] - 1 (high).
+  uint32_t* CF[9];
   float* J;
   float* Jp;
   float* mass;

@@ -211,28 +211,6 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 
 namespace aemh {
 
-// C and F accessors honouring the storage mode (0 FP32, 1 BF16, 2 FP16;
-// F is stored as F - I in 16-bit modes).
-__device__ __forceinline__ void put_c(PBuf& P, uint32_t j, int k, float c, int storage) {
-  if (storage == 0) P.C[k][j] = c;
-  else P.Ch[k][j] = storage == 1 ? f32_to_bf16(c) : f32_to_f16(c);
-}
-__device__ __forceinline__ void put_f(PBuf& P, uint32_t j, int k, float f, int storage) {
-  if (storage == 0) P.F[k][j] = f;
-  else {
-    float d = f - ((k % 4 == 0) ? 1.0f : 0.0f);
-    P.Fh[k][j] = storage == 1 ? f32_to_bf16(d) : f32_to_f16(d);
-  }
-}
-__device__ __forceinline__ float get_c(const PBuf& P, uint32_t j, int k, int storage) {
-  if (storage == 0) return P.C[k][j];
-  return storage == 1 ? bf16_to_f32(P.Ch[k][j]) : f16_to_f32(P.Ch[k][j]);
-}
-__device__ __forceinline__ float get_f(const PBuf& P, uint32_t j, int k, int storage) {
-  if (storage == 0) return P.F[k][j];
-  float d = storage == 1 ? bf16_to_f32(P.Fh[k][j]) : f16_to_f32(P.Fh[k][j]);
-  return d + ((k % 4 == 0) ? 1.0f : 0.0f);
-}
 // Slot addressing as uniform array base + 32-bit byte offset: the compiler
 // then emits global loads and stores with the base in SGPRs and one shared
 // VGPR offset, instead of a 64-bit address per access.  Working sets are
@@ -249,47 +227,102 @@ template <typename T>
 __device__ __forceinline__ T& at2(T* base, const Slot& s) {
   return *(T*)((char*)base + s.b2);
 }
-__device__ __forceinline__ void put_c(PBuf& P, const Slot& s, int k, float c, int storage) {
-  if (storage == 0) at4(P.C[k], s) = c;
-  else at2(P.Ch[k], s) = storage == 1 ? f32_to_bf16(c) : f32_to_f16(c);
+// Element j of a 4-byte per-slot array, by index or by Slot.
+template <typename T>
+__device__ __forceinline__ T& el4(T* base, uint32_t j) {
+  return base[j];
 }
-__device__ __forceinline__ void put_f(PBuf& P, const Slot& s, int k, float f, int storage) {
-  if (storage == 0) at4(P.F[k], s) = f;
-  else {
-    float d = f - ((k % 4 == 0) ? 1.0f : 0.0f);
-    at2(P.Fh[k], s) = storage == 1 ? f32_to_bf16(d) : f32_to_f16(d);
-  }
+template <typename T>
+__device__ __forceinline__ T& el4(T* base, const Slot& s) {
+  return at4(base, s);
 }
-__device__ __forceinline__ float get_f(const PBuf& P, const Slot& s, int k, int storage) {
-  if (storage == 0) return at4(P.F[k], s);
-  const uint16_t h = at2(P.Fh[k], s);
-  float d = storage == 1 ? bf16_to_f32(h) : f16_to_f32(h);
-  return d + ((k % 4 == 0) ? 1.0f : 0.0f);
+
+// C and F accessors honouring the storage mode (0 FP32, 1 BF16, 2 FP16).
+// The 16-bit modes store F - I and pack two values per word (PBuf::CF), so
+// a matrix is nine 32-bit loads or stores in every mode but FP32's 18.
+__device__ __forceinline__ uint32_t enc16(float v, int storage) {
+  return storage == 1 ? (uint32_t)f32_to_bf16(v) : (uint32_t)f32_to_f16(v);
 }
-__device__ __forceinline__ void copy_cf(PBuf& D, const Slot& o, const PBuf& S, const Slot& p,
-                                        int storage) {
+__device__ __forceinline__ float dec_lo(uint32_t w, int storage) {
+  return storage == 1 ? __uint_as_float(w << 16) : f16_to_f32((uint16_t)w);
+}
+__device__ __forceinline__ float dec_hi(uint32_t w, int storage) {
+  return storage == 1 ? __uint_as_float(w & 0xffff0000u) : f16_to_f32((uint16_t)(w >> 16));
+}
+__device__ __forceinline__ uint32_t pack16(float lo, float hi, int storage) {
+  return enc16(lo, storage) | (enc16(hi, storage) << 16);
+}
+__device__ __forceinline__ float eye3(int k) { return (k % 4 == 0) ? 1.0f : 0.0f; }
+
+template <class I>
+__device__ __forceinline__ void load_f(const PBuf& P, I j, float f[9], int storage) {
+  if (storage == 0) {
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
-    if (storage == 0) {
-      at4(D.C[k], o) = at4(S.C[k], p);
-      at4(D.F[k], o) = at4(S.F[k], p);
-    } else {
-      at2(D.Ch[k], o) = at2(S.Ch[k], p);
-      at2(D.Fh[k], o) = at2(S.Fh[k], p);
-    }
+    for (int k = 0; k < 9; k++) f[k] = el4(P.F[k], j);
+    return;
   }
+#pragma unroll
+  for (int w = 0; w < 4; w++) {
+    const uint32_t x = el4(P.CF[4 + w], j);
+    f[2 * w] = dec_lo(x, storage) + eye3(2 * w);
+    f[2 * w + 1] = dec_hi(x, storage) + eye3(2 * w + 1);
+  }
+  f[8] = dec_hi(el4(P.CF[8], j), storage) + 1.0f;
+}
+template <class I>
+__device__ __forceinline__ void load_cf(const PBuf& P, I j, float c[9], float f[9], int storage) {
+  if (storage == 0) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      c[k] = el4(P.C[k], j);
+      f[k] = el4(P.F[k], j);
+    }
+    return;
+  }
+#pragma unroll
+  for (int w = 0; w < 4; w++) {
+    const uint32_t x = el4(P.CF[w], j);
+    c[2 * w] = dec_lo(x, storage);
+    c[2 * w + 1] = dec_hi(x, storage);
+  }
+  load_f(P, j, f, storage);
+  c[8] = dec_lo(el4(P.CF[8], j), storage);
+}
+// C goes out first.  In the 16-bit modes its last component shares word 8
+// with F[8], so put_c9 writes words 0-3 and put_f9 (given c[8]) the rest.
+template <class I>
+__device__ __forceinline__ void put_c9(PBuf& P, I j, const float c[9], int storage) {
+  if (storage == 0) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) el4(P.C[k], j) = c[k];
+    return;
+  }
+#pragma unroll
+  for (int w = 0; w < 4; w++) el4(P.CF[w], j) = pack16(c[2 * w], c[2 * w + 1], storage);
+}
+template <class I>
+__device__ __forceinline__ void put_f9(PBuf& P, I j, const float f[9], float c8, int storage) {
+  if (storage == 0) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) el4(P.F[k], j) = f[k];
+    return;
+  }
+#pragma unroll
+  for (int w = 0; w < 4; w++)
+    el4(P.CF[4 + w], j) = pack16(f[2 * w] - eye3(2 * w), f[2 * w + 1] - eye3(2 * w + 1), storage);
+  el4(P.CF[8], j) = pack16(c8, f[8] - 1.0f, storage);
 }
 // C and F unchanged from slot p of S to slot j of D, copied in their stored
 // form (no 16-bit round trip through FP32).
-__device__ __forceinline__ void copy_cf(PBuf& D, uint32_t j, const PBuf& S, uint32_t p, int storage) {
+template <class I>
+__device__ __forceinline__ void copy_cf(PBuf& D, I j, const PBuf& S, I p, int storage) {
 #pragma unroll
   for (int k = 0; k < 9; k++) {
     if (storage == 0) {
-      D.C[k][j] = S.C[k][p];
-      D.F[k][j] = S.F[k][p];
+      el4(D.C[k], j) = el4(S.C[k], p);
+      el4(D.F[k], j) = el4(S.F[k], p);
     } else {
-      D.Ch[k][j] = S.Ch[k][p];
-      D.Fh[k][j] = S.Fh[k][p];
+      el4(D.CF[k], j) = el4(S.CF[k], p);
     }
   }
 }

@@ -246,10 +246,13 @@ __global__ __launch_bounds__(256) void k_gather_work(PBuf canon, PBuf work, cons
     work.x[a][j] = canon.x[a][p];
     work.v[a][j] = canon.v[a][p];
   }
+  float c[9], f[9];
   for (int k = 0; k < 9; k++) {
-    put_c(work, j, k, canon.C[k][p], storage);
-    put_f(work, j, k, canon.F[k][p], storage);
+    c[k] = canon.C[k][p];
+    f[k] = canon.F[k][p];
   }
+  put_c9(work, j, c, storage);
+  put_f9(work, j, f, c[8], storage);
   work.J[j] = canon.J[p];
   work.Jp[j] = canon.Jp[p];
   work.mass[j] = canon.mass[p];
@@ -266,9 +269,11 @@ __global__ __launch_bounds__(256) void k_scatter_work(PBuf work, PBuf canon, con
     canon.x[a][p] = work.x[a][j];
     canon.v[a][p] = work.v[a][j];
   }
+  float c[9], f[9];
+  load_cf(work, j, c, f, storage);
   for (int k = 0; k < 9; k++) {
-    canon.C[k][p] = get_c(work, j, k, storage);
-    canon.F[k][p] = get_f(work, j, k, storage);
+    canon.C[k][p] = c[k];
+    canon.F[k][p] = f[k];
   }
   canon.J[p] = work.J[j];
   canon.Jp[p] = work.Jp[j];

@@ -291,11 +291,7 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
     }
     if (full) {
       float F[9];
-#pragma unroll
-      for (int k = 0; k < 9; k++) {
-        q.c[k] = get_c(P, p, k, STORE);
-        F[k] = get_f(P, p, k, STORE);
-      }
+      load_cf(P, p, q.c, F, STORE);
       const uint32_t mi = P.mat[p];
       const MatDev M = mats[mi < (uint32_t)nmat ? mi : (uint32_t)nmat];
       float stress[9];
@@ -684,8 +680,7 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
     pid = at4(cur.id, ps);
     pjp = at4(cur.Jp, ps);
     if (!S.vel_only) {
-#pragma unroll
-      for (int k = 0; k < 9; k++) F[k] = get_f(cur, ps, k, STORE);
+      load_f(cur, ps, F, STORE);
     }
   }
   const Stencil st = make_stencil(S, px, py, pz);
@@ -1068,8 +1063,7 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
   at4(nxt.v[0], os) = nv[0] * S.damping;
   at4(nxt.v[1], os) = nv[1] * S.damping;
   at4(nxt.v[2], os) = nv[2] * S.damping;
-#pragma unroll
-  for (int k = 0; k < 9; k++) put_c(nxt, os, k, nc[k], STORE);
+  put_c9(nxt, os, nc, STORE);
   // Q9: F' = (I + dt grad v) F; validity; J = clamp(det F', 0.1, 10).
   float nf[9];
 #pragma unroll
@@ -1101,8 +1095,7 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
     identity3(nf);
     fj = 1.0f;
   }
-#pragma unroll
-  for (int k = 0; k < 9; k++) put_f(nxt, os, k, nf[k], STORE);
+  put_f9(nxt, os, nf, nc[8], STORE);
   at4(nxt.J, os) = fj;
   at4(nxt.Jp, os) = jp;
 }
