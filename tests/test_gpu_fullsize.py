@@ -1,5 +1,6 @@
 """GPU: the full-size workloads (1,048,576-particle snow dam break on 128^3,
-16,777,216 particles in three boxes on a sparse 256^3 grid), checked through
+16,777,216 particles in three boxes on a sparse 256^3 grid, and the
+all-integer MpmSimulation on 1M and 16M particles), checked through
 properties that hold at any size.
 
 The oracle needs minutes per step at these sizes, so these tests do not
@@ -66,3 +67,33 @@ def test_c3_full_size_q32_order_and_shape_invariant():
     b = run(wl, 2, shape=(2, 1, 128, 4096), perm=perm)
     assert np.isfinite(a["pos"]).all()
     assert_same(a, b, perm, "c3 16.8M")
+
+
+def fixed_run(sc, steps, perm=None):
+    pos, mass = (sc.pos, sc.mass) if perm is None else (sc.pos[perm], sc.mass[perm])
+    sim = aemh.FixedSim(sc.config)
+    sim.add_particles(pos, np.zeros_like(pos), mass)
+    sim.step(steps)
+    d, st = sim.read(), sim.state()
+    sim.close()
+    return d, st
+
+
+@pytest.mark.parametrize("n,res,steps", [(1 << 20, 64, 6), (1 << 24, 160, 5)])
+def test_fixed_point_full_size_order_invariant(n, res, steps):
+    """The all-integer MpmSimulation at 1M and 16M particles.  Its P2G adds
+    saturating Q32.32 sums; while no cell can clip (the scatter bound, which
+    the run reports), they are plain integer sums, so permuting the particles
+    before they are added leaves every raw field of every particle unchanged.
+    Both runs cross a re-sort of the cell order (every 4th substep)."""
+    sc = W.fixed_cloud(n, res)
+    perm = np.random.default_rng(7).permutation(n)
+    a, sa = fixed_run(sc, steps)
+    b, sb = fixed_run(sc, steps, perm)
+    assert sa["serial_substeps"] == 0 and sb["serial_substeps"] == 0
+    assert sa["step_count"] == sb["step_count"] == steps
+    nd = int((a["q"][perm] != b["q"]).sum())
+    assert nd == 0, f"{nd} raw values differ"
+    assert np.array_equal(a["flags"][perm], b["flags"])
+    # The cloud moved: positions changed from the upload.
+    assert int((a["q"][:, 0:3] != sc.pos).sum()) > 0
