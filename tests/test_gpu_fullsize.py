@@ -1,7 +1,7 @@
 """GPU: the full-size workloads (1,048,576-particle snow dam break on 128^3,
-16,777,216 particles in three boxes on a sparse 256^3 grid, and the
-all-integer MpmSimulation on 1M and 16M particles), checked through
-properties that hold at any size.
+16,777,216 particles in three boxes on a sparse 256^3 grid, 256 batched
+scenes of 15.9M particles in all, and the all-integer MpmSimulation on 1M
+and 16M particles), checked through properties that hold at any size.
 
 The oracle needs minutes per step at these sizes, so these tests do not
 compare against it; the bit-exact comparisons with the oracle run at scaled
@@ -97,3 +97,30 @@ def test_fixed_point_full_size_order_invariant(n, res, steps):
     assert np.array_equal(a["flags"][perm], b["flags"])
     # The cloud moved: positions changed from the upload.
     assert int((a["q"][:, 0:3] != sc.pos).sum()) > 0
+
+
+def test_c4_full_size_batched_equals_plain_q32():
+    """All 256 scenes (log-uniform 4K..256K particles, grids 32^3..128^3,
+    15.9M particles) advanced together by the batched step equal, scene by
+    scene, 256 plain single-scene engines run on the same particles."""
+    wl = W.c4_batched_scenes()
+    wl.accum = aemh.ACCUM_Q32_32
+    assert len(wl.scenes) == 256
+    eng = aemh.Engine()
+    W.load_engine(eng, wl)
+    eng.batch_step(3)
+    d = eng.download()
+    eng.close()
+    for k, sc in enumerate(wl.scenes):
+        sl = slice(sc.first, sc.first + sc.count)
+        e = aemh.Engine()
+        e.set_config(wl.config())
+        e.set_grid(sc.nx, sc.ny, sc.nz, sc.dx, tuple(sc.origin))
+        e.set_materials(wl.materials)
+        e.upload(wl.pos[sl], mass=wl.mass[sl], mat=wl.mat[sl])
+        for _ in range(3):
+            e.step(sc.dt)
+        p = e.download()
+        e.close()
+        for f in FIELDS:
+            assert np.array_equal(d[f][sl].view(np.uint32), p[f].view(np.uint32)), (k, f)
