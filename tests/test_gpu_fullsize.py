@@ -12,6 +12,8 @@ into workgroups, nor on which P2G path (LDS tile or direct HBM atomic) a
 particle takes, so a run from a permuted upload under a different launch
 shape must reproduce every field of every particle bit for bit.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -124,3 +126,69 @@ def test_c4_full_size_batched_equals_plain_q32():
         e.close()
         for f in FIELDS:
             assert np.array_equal(d[f][sl].view(np.uint32), p[f].view(np.uint32)), (k, f)
+
+
+# ---- FP32 mode against the oracle at full size ------------------------------
+# The oracle (multi-threaded, with the reference's per-thread grids and
+# merge) runs C2 in about half a second per step and C3 in about five on 16
+# host threads, so short full-size trajectories are compared directly.
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _oracle(wl):
+    from oracle import refcpu as R
+    nx, ny, nz, dx, o = wl.grid
+    cfg = R.make_config(nx, ny, nz, dx, o, 1, wl.gravity, 0.995, wl.bc_margin, 0.0, R.ACCUM_F32,
+                        R.CLAMP_AUTO, threads=min(16, os.cpu_count() or 1))
+    p, m, e = aemh.material_arrays(wl.materials)
+    return R, cfg, R.Materials(p, m, e)
+
+
+def test_c2_full_size_fp32_vs_oracle_20_steps():
+    """1,048,576 snow particles, 20 FP32 steps.  The bar is three times the
+    oracle's own reorder noise (the oracle re-run with its P2G order
+    reshuffled every step, as the reference's rayon merge order varies), and
+    1e-4 relative L2 on positions and velocities in any case."""
+    wl = W.c2_snow_dam()
+    R, cfg, M = _oracle(wl)
+    eng = aemh.Engine()
+    W.load_engine(eng, wl)
+    base = R.State(wl.pos, mass=wl.mass, mat=wl.mat)
+    rerun = R.State(wl.pos, mass=wl.mass, mat=wl.mat)
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        eng.step(wl.dt)
+        R.step(cfg, M, base, wl.dt)
+        R.step(cfg, M, rerun, wl.dt, idx=rng.permutation(wl.n).astype(np.uint32))
+    d = eng.download()
+    eng.close()
+    for f, floor_min in (("pos", 1e-7), ("vel", 1e-6), ("F", 1e-6), ("C", 1e-6)):
+        got = _rel(d[f], getattr(base, f))
+        floor = _rel(getattr(rerun, f), getattr(base, f))
+        print(f"c2 fp32 20 steps {f}: gpu {got:.3e}, reorder noise {floor:.3e}", flush=True)
+        assert got <= max(3.0 * floor, floor_min), (f, got, floor)
+        if f in ("pos", "vel"):
+            assert got <= 1e-4
+
+
+def test_c3_full_size_fp32_vs_oracle_3_steps():
+    """16,777,216 particles (fluid, jelly and snow boxes on the sparse 256^3
+    grid), 3 FP32 steps against the oracle."""
+    wl = W.c3_three_boxes()
+    R, cfg, M = _oracle(wl)
+    eng = aemh.Engine()
+    W.load_engine(eng, wl)
+    base = R.State(wl.pos, mass=wl.mass, mat=wl.mat)
+    for _ in range(3):
+        eng.step(wl.dt)
+        R.step(cfg, M, base, wl.dt)
+    d = eng.download()
+    eng.close()
+    for f, bar in (("pos", 1e-6), ("vel", 1e-5), ("F", 1e-5), ("C", 3e-5), ("J", 1e-5)):
+        got = _rel(d[f], getattr(base, f))
+        print(f"c3 fp32 3 steps {f}: gpu {got:.3e}", flush=True)
+        assert got <= bar, (f, got)
