@@ -590,10 +590,9 @@ void enqueue_substep(aemh_ctx* c, int cur, bool resort) {
   const int storage = c->work[0].storage;
   const int q32 = c->cfg.accum == AEMH_ACCUM_Q32_32;
   hipStream_t st = c->stream;
-  timed(c, "step_begin", [&] { launch_step_begin(st, c->sort); });
   timed(c, "count", [&] { launch_count(st, a, c->scd_d, ns, n, c->sort, c->nb_total); });
   timed(c, "scan", [&] { launch_scan(st, c->scd_d, ns, c->sort, c->nb_total); });
-  timed(c, "scatter", [&] { launch_scatter(st, n, c->sort); });
+  timed(c, "scatter", [&] { launch_scatter(st, n, c->nb_total, c->sort); });
   timed(c, "clear", [&] { launch_clear(st, c->sort, c->grid, q32, c->shape.upd_wgs); });
   if (q32) {
     timed(c, "p2g", [&] {
@@ -1261,10 +1260,9 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   const int q32 = c->cfg.accum == AEMH_ACCUM_Q32_32;
   hipStream_t st = c->stream;
   LaunchShape& ls = c->shape;
-  launch_step_begin(st, c->sort);
   launch_count(st, a, c->scd_d, ns, n, c->sort, c->nb_total);
   launch_scan(st, c->scd_d, ns, c->sort, c->nb_total);
-  launch_scatter(st, n, c->sort);
+  launch_scatter(st, n, c->nb_total, c->sort);
   CHECK_LAUNCH(c);
   constexpr int kReps = 3;
   float best = 1e30f, ms = 0.0f;

@@ -8,11 +8,6 @@
 namespace aemh {
 namespace {
 
-// Per-substep counters (not the sticky overflow flag).
-__global__ void k_step_begin(SortDev s) {
-  if (threadIdx.x < kCntOverflow) s.counters[threadIdx.x] = 0;
-}
-
 // Runs of equal keys seen by one wave (how far the working set is from block
 // order), spread over kRunLanes counters on separate 64 B lines.
 __device__ __forceinline__ void count_runs(SortDev& s, uint32_t wave_id, uint32_t runs) {
@@ -159,10 +154,14 @@ __global__ __launch_bounds__(256) void k_scan_tiles(const SceneDev* __restrict__
   if (threadIdx.x == 255) s.tsum[blockIdx.x] = wpre + incl;
 }
 
+// Exclusive scan of the tile sums (one workgroup).  It also starts the
+// substep's bookkeeping: the per-substep counters (not the sticky overflow
+// flag) are zeroed here, before the scatter counts the active blocks.
 __global__ __launch_bounds__(256) void k_scan_tsum(SortDev s, uint32_t ntiles) {
   __shared__ uint32_t wtot[4];
   __shared__ uint32_t carry_s;
   if (threadIdx.x == 0) carry_s = 0;
+  if (threadIdx.x < kCntOverflow) s.counters[threadIdx.x] = 0u;
   // The count kernel's wave runs: total to the host-mapped word, counters
   // cleared for the next substep.
   if (threadIdx.x < 64) {
@@ -193,30 +192,29 @@ __global__ __launch_bounds__(256) void k_scan_tsum(SortDev s, uint32_t ntiles) {
   }
 }
 
-// Compact the activation flags into the active-block list (one atomic per
-// wave) and clear them for the next substep.
-__global__ __launch_bounds__(256) void k_compact_active(SortDev s, uint32_t nb) {
-  const uint32_t b = blockIdx.x * 256u + threadIdx.x;
-  const bool on = b < nb && s.flag[b] != 0u;
-  const unsigned long long m = __ballot(on);
-  if (m == 0) return;
-  const int lane = __lane_id();
-  const int leader = __ffsll((long long)m) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(&s.counters[0], (uint32_t)__popcll(m));
-  base = __shfl(base, leader, 64);
-  if (on) {
-    const unsigned long long lt = (1ull << lane) - 1ull;
-    s.active[base + (uint32_t)__popcll(m & lt)] = b;
-    s.flag[b] = 0u;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_scatter(uint32_t n, SortDev s) {
+// Particle slots into block order (perm), fused with the compaction of the
+// activation flags into the active-block list (one atomic per wave; flags
+// cleared for the next substep).  Both only need the finished scans.
+__global__ __launch_bounds__(256) void k_scatter(uint32_t n, uint32_t nb, SortDev s) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t k = s.key[i];
-  s.perm[s.off[k] + s.tsum[k / kScanTile] + s.rank[i]] = i;
+  const bool on = i < nb && s.flag[i] != 0u;
+  const unsigned long long m = __ballot(on);
+  if (m) {  // wave-uniform
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&s.counters[kCntActive], (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (on) {
+      const unsigned long long lt = (1ull << lane) - 1ull;
+      s.active[base + (uint32_t)__popcll(m & lt)] = i;
+      s.flag[i] = 0u;
+    }
+  }
+  if (i < n) {
+    const uint32_t k = s.key[i];
+    s.perm[s.off[k] + s.tsum[k / kScanTile] + s.rank[i]] = i;
+  }
 }
 
 // Zero the accumulators of this substep's active blocks (one wave per block).
@@ -366,7 +364,6 @@ inline unsigned grid_for(uint32_t n, unsigned wg) { return (unsigned)((n + wg - 
 
 }  // namespace
 
-void launch_step_begin(hipStream_t st, SortDev s) { k_step_begin<<<1, 64, 0, st>>>(s); }
 void launch_count(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, uint32_t n, SortDev s,
                   uint32_t nb_total) {
   if (!n) return;
@@ -379,10 +376,10 @@ void launch_scan(hipStream_t st, const SceneDev* sc, int ns, SortDev s, uint32_t
   const uint32_t tiles = (nb + kScanTile - 1) / kScanTile;
   k_scan_tiles<<<tiles, 256, 0, st>>>(sc, ns, s, nb);
   k_scan_tsum<<<1, 256, 0, st>>>(s, tiles);
-  k_compact_active<<<grid_for(nb, 256), 256, 0, st>>>(s, nb);
 }
-void launch_scatter(hipStream_t st, uint32_t n, SortDev s) {
-  if (n) k_scatter<<<grid_for(n, 256), 256, 0, st>>>(n, s);
+void launch_scatter(hipStream_t st, uint32_t n, uint32_t nb, SortDev s) {
+  const uint32_t m = n > nb ? n : nb;
+  if (m) k_scatter<<<grid_for(m, 256), 256, 0, st>>>(n, nb, s);
 }
 void launch_clear(hipStream_t st, SortDev s, GridDev g, int q32, int upd_wgs) {
   k_clear<<<upd_wgs, 256, 0, st>>>(s, g, q32);
