@@ -592,8 +592,7 @@ void enqueue_substep(aemh_ctx* c, int cur, bool resort) {
   hipStream_t st = c->stream;
   timed(c, "count", [&] { launch_count(st, a, c->scd_d, ns, n, c->sort, c->nb_total); });
   timed(c, "scan", [&] { launch_scan(st, c->scd_d, ns, c->sort, c->nb_total); });
-  timed(c, "scatter", [&] { launch_scatter(st, n, c->nb_total, c->sort); });
-  timed(c, "clear", [&] { launch_clear(st, c->sort, c->grid, q32, c->shape.upd_wgs); });
+  timed(c, "scatter", [&] { launch_scatter(st, n, c->nb_total, c->sort, c->grid, q32); });
   if (q32) {
     timed(c, "p2g", [&] {
       det::launch_p2g(st, c->shape, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
@@ -1262,7 +1261,7 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   LaunchShape& ls = c->shape;
   launch_count(st, a, c->scd_d, ns, n, c->sort, c->nb_total);
   launch_scan(st, c->scd_d, ns, c->sort, c->nb_total);
-  launch_scatter(st, n, c->nb_total, c->sort);
+  launch_scatter(st, n, c->nb_total, c->sort, c->grid, q32);
   CHECK_LAUNCH(c);
   constexpr int kReps = 3;
   float best = 1e30f, ms = 0.0f;

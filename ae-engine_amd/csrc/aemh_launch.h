@@ -24,7 +24,7 @@ void launch_count(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, uint32_t
                   uint32_t nb_total);
 // Scan of the block counts; also builds this substep's active-block list.
 void launch_scan(hipStream_t st, const SceneDev* sc, int ns, SortDev s, uint32_t nb);
-void launch_scatter(hipStream_t st, uint32_t n, uint32_t nb, SortDev s);
+void launch_scatter(hipStream_t st, uint32_t n, uint32_t nb, SortDev s, GridDev g, int q32);
 void launch_clear(hipStream_t st, SortDev s, GridDev g, int q32, int upd_wgs);
 // canonical (capacity) <-> working set
 void launch_gather_work(hipStream_t st, PBuf canon, PBuf work, const uint32_t* idx, uint32_t n,

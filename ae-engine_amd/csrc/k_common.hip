@@ -194,8 +194,11 @@ __global__ __launch_bounds__(256) void k_scan_tsum(SortDev s, uint32_t ntiles) {
 
 // Particle slots into block order (perm), fused with the compaction of the
 // activation flags into the active-block list (one atomic per wave; flags
-// cleared for the next substep).  Both only need the finished scans.
-__global__ __launch_bounds__(256) void k_scatter(uint32_t n, uint32_t nb, SortDev s) {
+// cleared for the next substep) and the zeroing of each listed block's
+// accumulators, which the wave does one block at a time with coalesced
+// stores.  All of it only needs the finished scans.
+__global__ __launch_bounds__(256) void k_scatter(uint32_t n, uint32_t nb, SortDev s, GridDev g,
+                                                 int q32) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   const bool on = i < nb && s.flag[i] != 0u;
   const unsigned long long m = __ballot(on);
@@ -209,6 +212,15 @@ __global__ __launch_bounds__(256) void k_scatter(uint32_t n, uint32_t nb, SortDe
       const unsigned long long lt = (1ull << lane) - 1ull;
       s.active[base + (uint32_t)__popcll(m & lt)] = i;
       s.flag[i] = 0u;
+    }
+    const uint32_t b0 = i - (uint32_t)lane;  // block of lane 0
+    for (unsigned long long r = m; r; r &= r - 1ull) {
+      const uint32_t node = ((b0 + (uint32_t)(__ffsll((long long)r) - 1)) << 6) | (uint32_t)lane;
+      if (q32) {
+        for (int f = 0; f < kDetFields; f++) g.accq[(size_t)f * g.nodes + node] = 0;
+      } else {
+        for (int f = 0; f < 7; f++) g.accf[(size_t)f * g.nodes + node] = 0.0f;
+      }
     }
   }
   if (i < n) {
@@ -377,9 +389,9 @@ void launch_scan(hipStream_t st, const SceneDev* sc, int ns, SortDev s, uint32_t
   k_scan_tiles<<<tiles, 256, 0, st>>>(sc, ns, s, nb);
   k_scan_tsum<<<1, 256, 0, st>>>(s, tiles);
 }
-void launch_scatter(hipStream_t st, uint32_t n, uint32_t nb, SortDev s) {
+void launch_scatter(hipStream_t st, uint32_t n, uint32_t nb, SortDev s, GridDev g, int q32) {
   const uint32_t m = n > nb ? n : nb;
-  if (m) k_scatter<<<grid_for(m, 256), 256, 0, st>>>(n, nb, s);
+  if (m) k_scatter<<<grid_for(m, 256), 256, 0, st>>>(n, nb, s, g, q32);
 }
 void launch_clear(hipStream_t st, SortDev s, GridDev g, int q32, int upd_wgs) {
   k_clear<<<upd_wgs, 256, 0, st>>>(s, g, q32);

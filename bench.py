@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # per active grid node: P2G writes 28 B (m, p, f); update reads 28, writes
 #   16 (v, m); G2P reads 16 -> step 88 B; P2G 28 B; G2P 16 B.
 BYTES = {"p2g": (110, 28), "g2p": (156, 16), "grid_update": (0, 44), "step": (214, 88)}
-KERNELS = ["count", "scan", "scatter", "clear", "p2g", "grid_update", "g2p"]
+KERNELS = ["count", "scan", "scatter", "p2g", "grid_update", "g2p"]
 # HBM bytes per dispatch from rocprofv3 FETCH_SIZE (x 2.000) and WRITE_SIZE
 # (x 1.000), calibrated against a 256 MiB copy, as recorded in
 # profiles/c2_snow_1m_hbm_traffic.md (1M-particle snow workload) and
@@ -226,7 +226,7 @@ def extra_batched(W, aemh, steps=5):
             "batched_vs_plain_speedup": el_p / el_b}
 
 
-TIER_KERNELS = ("lod_classify", "lod_select", "near_cull", "count", "scan", "scatter", "clear",
+TIER_KERNELS = ("lod_classify", "lod_select", "near_cull", "count", "scan", "scatter",
                 "p2g", "grid_update", "g2p", "far_update", "lifetimes")
 
 
