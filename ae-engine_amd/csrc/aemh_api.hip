@@ -469,6 +469,27 @@ void timed(aemh_ctx* c, const char* name, Fn&& fn) {
   c->pending.push_back({name, {a, b}});
 }
 
+// timed() for a stage whose kernels launch through launch_k: the events ride
+// on the kernels' own dispatches.  A stage that launched nothing (no
+// particles) records an empty pair.
+template <class Fn>
+void timed_k(aemh_ctx* c, const char* name, Fn&& fn) {
+  if (!c->prof) {
+    fn();
+    return;
+  }
+  hipEvent_t a = get_event(c), b = get_event(c);
+  g_stage = StageEvents{a, b, false};
+  fn();
+  const bool launched = g_stage.launched;
+  g_stage = StageEvents{};
+  if (!launched) {
+    (void)hipEventRecord(a, c->stream);
+    (void)hipEventRecord(b, c->stream);
+  }
+  c->pending.push_back({name, {a, b}});
+}
+
 void collect_profile(aemh_ctx* c) {
   for (auto& pr : c->pending) {
     float ms = 0;
@@ -590,23 +611,23 @@ void enqueue_substep(aemh_ctx* c, int cur, bool resort) {
   const int storage = c->work[0].storage;
   const int q32 = c->cfg.accum == AEMH_ACCUM_Q32_32;
   hipStream_t st = c->stream;
-  timed(c, "count", [&] { launch_count(st, a, c->scd_d, ns, n, c->sort, c->nb_total); });
-  timed(c, "scan", [&] { launch_scan(st, c->scd_d, ns, c->sort, c->nb_total); });
-  timed(c, "scatter", [&] { launch_scatter(st, n, c->nb_total, c->sort, c->grid, q32); });
+  timed_k(c, "count", [&] { launch_count(st, a, c->scd_d, ns, n, c->sort, c->nb_total); });
+  timed_k(c, "scan", [&] { launch_scan(st, c->scd_d, ns, c->sort, c->nb_total); });
+  timed_k(c, "scatter", [&] { launch_scatter(st, n, c->nb_total, c->sort, c->grid, q32); });
   if (q32) {
-    timed(c, "p2g", [&] {
+    timed_k(c, "p2g", [&] {
       det::launch_p2g(st, c->shape, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
     });
-    timed(c, "grid_update", [&] { det::launch_grid_update(st, c->shape, c->scd_d, ns, c->sort, c->grid, n, c->nb_total); });
-    timed(c, "g2p", [&] {
+    timed_k(c, "grid_update", [&] { det::launch_grid_update(st, c->shape, c->scd_d, ns, c->sort, c->grid, n, c->nb_total); });
+    timed_k(c, "g2p", [&] {
       det::launch_g2p(st, c->shape, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, inplace);
     });
   } else {
-    timed(c, "p2g", [&] {
+    timed_k(c, "p2g", [&] {
       fast::launch_p2g(st, c->shape, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
     });
-    timed(c, "grid_update", [&] { fast::launch_grid_update(st, c->shape, c->scd_d, ns, c->sort, c->grid, n, c->nb_total); });
-    timed(c, "g2p", [&] {
+    timed_k(c, "grid_update", [&] { fast::launch_grid_update(st, c->shape, c->scd_d, ns, c->sort, c->grid, n, c->nb_total); });
+    timed_k(c, "g2p", [&] {
       fast::launch_g2p(st, c->shape, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, inplace);
     });
   }

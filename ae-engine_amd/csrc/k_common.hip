@@ -376,22 +376,24 @@ inline unsigned grid_for(uint32_t n, unsigned wg) { return (unsigned)((n + wg - 
 
 }  // namespace
 
+thread_local StageEvents g_stage;
+
 void launch_count(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, uint32_t n, SortDev s,
                   uint32_t nb_total) {
   if (!n) return;
   if (nb_total <= kCountLdsBins)
-    k_count_lds<<<grid_for(n, 256 * kCountRounds), 256, 0, st>>>(cur, sc, ns, n, s, nb_total);
+    launch_k(k_count_lds, grid_for(n, 256 * kCountRounds), 256, 0, st, cur, sc, ns, n, s, nb_total);
   else
-    k_count<<<grid_for(n, 256), 256, 0, st>>>(cur, sc, ns, n, s);
+    launch_k(k_count, grid_for(n, 256), 256, 0, st, cur, sc, ns, n, s);
 }
 void launch_scan(hipStream_t st, const SceneDev* sc, int ns, SortDev s, uint32_t nb) {
   const uint32_t tiles = (nb + kScanTile - 1) / kScanTile;
-  k_scan_tiles<<<tiles, 256, 0, st>>>(sc, ns, s, nb);
-  k_scan_tsum<<<1, 256, 0, st>>>(s, tiles);
+  launch_k(k_scan_tiles, tiles, 256, 0, st, sc, ns, s, nb);
+  launch_k(k_scan_tsum, 1, 256, 0, st, s, tiles);
 }
 void launch_scatter(hipStream_t st, uint32_t n, uint32_t nb, SortDev s, GridDev g, int q32) {
   const uint32_t m = n > nb ? n : nb;
-  if (m) k_scatter<<<grid_for(m, 256), 256, 0, st>>>(n, nb, s, g, q32);
+  if (m) launch_k(k_scatter, grid_for(m, 256), 256, 0, st, n, nb, s, g, q32);
 }
 void launch_clear(hipStream_t st, SortDev s, GridDev g, int q32, int upd_wgs) {
   k_clear<<<upd_wgs, 256, 0, st>>>(s, g, q32);

@@ -679,8 +679,23 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
     pmass = at4(cur.mass, ps);
     pid = at4(cur.id, ps);
     pjp = at4(cur.Jp, ps);
+    // C and F are read and written out here, not through the PBuf
+    // accessors: with those, the compiler kept the plane pointers of both
+    // buffers live in SGPRs and spilled them to VGPR lanes (122 v_readlane
+    // in the FP32 kernel, 254 in the BF16 one), which cost 3 % of G2P time.
     if (!S.vel_only) {
-      load_f(cur, ps, F, STORE);
+      if constexpr (STORE == 0) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) F[k] = at4(cur.F[k], ps);
+      } else {
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+          const uint32_t x = at4(cur.CF[4 + w], ps);
+          F[2 * w] = dec_lo(x, STORE) + eye3(2 * w);
+          F[2 * w + 1] = dec_hi(x, STORE) + eye3(2 * w + 1);
+        }
+        F[8] = dec_hi(at4(cur.CF[8], ps), STORE) + 1.0f;
+      }
     }
   }
   const Stencil st = make_stencil(S, px, py, pz);
@@ -1063,7 +1078,13 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
   at4(nxt.v[0], os) = nv[0] * S.damping;
   at4(nxt.v[1], os) = nv[1] * S.damping;
   at4(nxt.v[2], os) = nv[2] * S.damping;
-  put_c9(nxt, os, nc, STORE);
+  if constexpr (STORE == 0) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) at4(nxt.C[k], os) = nc[k];
+  } else {  // words 0-3; C[8] shares word 8 with F[8], written below
+#pragma unroll
+    for (int w = 0; w < 4; w++) at4(nxt.CF[w], os) = pack16(nc[2 * w], nc[2 * w + 1], STORE);
+  }
   // Q9: F' = (I + dt grad v) F; validity; J = clamp(det F', 0.1, 10).
   float nf[9];
 #pragma unroll
@@ -1095,7 +1116,15 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
     identity3(nf);
     fj = 1.0f;
   }
-  put_f9(nxt, os, nf, nc[8], STORE);
+  if constexpr (STORE == 0) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) at4(nxt.F[k], os) = nf[k];
+  } else {
+#pragma unroll
+    for (int w = 0; w < 4; w++)
+      at4(nxt.CF[4 + w], os) = pack16(nf[2 * w] - eye3(2 * w), nf[2 * w + 1] - eye3(2 * w + 1), STORE);
+    at4(nxt.CF[8], os) = pack16(nc[8], nf[8] - 1.0f, STORE);
+  }
   at4(nxt.J, os) = fj;
   at4(nxt.Jp, os) = jp;
 }
@@ -1107,11 +1136,11 @@ void launch_p2g_store(hipStream_t st, int tiles, int partials, unsigned grid, PB
                       const SceneDev* sc, int ns, const MatDev* mats, int nmat, uint32_t n,
                       SortDev s, GridDev g) {
   if (tiles <= 2)
-    k_p2g<STORE, 2><<<grid, 256, 0, st>>>(cur, sc, ns, mats, nmat, n, s, g, partials);
+    launch_k(k_p2g<STORE, 2>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
   else if (tiles == 3)
-    k_p2g<STORE, 3><<<grid, 256, 0, st>>>(cur, sc, ns, mats, nmat, n, s, g, partials);
+    launch_k(k_p2g<STORE, 3>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
   else
-    k_p2g<STORE, 4><<<grid, 256, 0, st>>>(cur, sc, ns, mats, nmat, n, s, g, partials);
+    launch_k(k_p2g<STORE, 4>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
 }
 
 void launch_p2g(hipStream_t st, const LaunchShape& ls, PBuf cur, const SceneDev* sc, int ns,
@@ -1128,8 +1157,8 @@ void launch_p2g(hipStream_t st, const LaunchShape& ls, PBuf cur, const SceneDev*
 
 void launch_grid_update(hipStream_t st, const LaunchShape& ls, const SceneDev* sc, int ns,
                         SortDev s, GridDev g, uint32_t n, uint32_t nb_total) {
-  k_grid_update<<<ls.upd_wgs, 256, 0, st>>>(sc, ns, s, g, ls.p2g_tiles, ls.p2g_partials, n,
-                                            nb_total);
+  launch_k(k_grid_update, ls.upd_wgs, 256, 0, st, sc, ns, s, g, ls.p2g_tiles, ls.p2g_partials, n,
+           nb_total);
 }
 
 void launch_g2p(hipStream_t st, const LaunchShape& ls, PBuf cur, PBuf nxt, const SceneDev* sc,
@@ -1140,11 +1169,11 @@ void launch_g2p(hipStream_t st, const LaunchShape& ls, PBuf cur, PBuf nxt, const
   const unsigned grid = (n + wg - 1u) / wg;
   const size_t lds = (size_t)g2p_tiles_for((int)wg) * kTileNodes * sizeof(float4);
   if (storage == 1)
-    k_g2p<1><<<grid, wg, lds, st>>>(cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
+    launch_k(k_g2p<1>, grid, wg, lds, st, cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
   else if (storage == 2)
-    k_g2p<2><<<grid, wg, lds, st>>>(cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
+    launch_k(k_g2p<2>, grid, wg, lds, st, cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
   else
-    k_g2p<0><<<grid, wg, lds, st>>>(cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
+    launch_k(k_g2p<0>, grid, wg, lds, st, cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
 }
 
 }  // namespace AEMH_NS
