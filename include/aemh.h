@@ -285,9 +285,14 @@ aemh_status aemh_conservation_f64(aemh_ctx* ctx, double* out5);
 aemh_status aemh_last_substep_info(aemh_ctx* ctx, uint32_t* active_blocks,
                                    uint32_t* particles);
 
-/* Per-kernel timing: while enabled, substeps launch eagerly with a HIP event
- * pair around each kernel section on the context stream (instead of the
- * graph replay); aemh_profile_get gives the average ms per section. */
+/* Per-kernel timing: while enabled, substeps launch eagerly on the context
+ * stream (instead of the graph replay) and each section's HIP event pair is
+ * recorded by its kernels' own dispatches (hipExtLaunchKernel start and stop
+ * events); aemh_profile_get gives the average ms per section.  Sections of
+ * a substep: "count", "scan", "scatter" (with the active-block list and the
+ * accumulator clear), "p2g", "grid_update", "g2p"; tier ticks add
+ * "lod_classify", "lod_select", "near_cull", "far_update", "lifetimes".
+ * No reference counterpart. */
 aemh_status aemh_profile_enable(aemh_ctx* ctx, int on);
 aemh_status aemh_profile_get(aemh_ctx* ctx, const char* kernel, double* avg_ms,
                              uint64_t* count);
