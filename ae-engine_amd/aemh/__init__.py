@@ -158,6 +158,7 @@ EXPORTS = [
     "aemh_fixed_last_error", "aemh_fixed_add_particle", "aemh_fixed_add_particles",
     "aemh_fixed_remove_particle", "aemh_fixed_step", "aemh_fixed_set_material_model",
     "aemh_fixed_enable_fracture", "aemh_fixed_disable_fracture", "aemh_fixed_get_state",
+    "aemh_fixed_set_sort_interval",
     "aemh_fixed_read_particles", "aemh_fixed_write_particles", "aemh_fixed_fracture_events",
     "aemh_fixed_get_stats", "aemh_fixed_snapshots",
 ]
@@ -240,6 +241,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.aemh_fixed_step.argtypes = [P, u32]
     lib.aemh_fixed_set_material_model.argtypes = [P, u32, C.c_int64]
     lib.aemh_fixed_enable_fracture.argtypes = [P, C.c_int64]
+    lib.aemh_fixed_set_sort_interval.argtypes = [P, u32]
     lib.aemh_fixed_disable_fracture.argtypes = [P]
     lib.aemh_fixed_get_state.argtypes = [P, C.POINTER(FixedState)]
     lib.aemh_fixed_read_particles.argtypes = [P, C.c_uint64, C.c_uint64, i64p,
@@ -714,6 +716,10 @@ class FixedSim:
 
     def set_model(self, model: int, param: int = 0):
         self._check(self._lib.aemh_fixed_set_material_model(self._h, model, int(param)))
+
+    def set_sort_interval(self, substeps: int):
+        """Re-sort interval of the cell-ordered P2G (default 1)."""
+        self._check(self._lib.aemh_fixed_set_sort_interval(self._h, int(substeps)))
 
     def set_fracture(self, on: bool, threshold: int = 0):
         if on:

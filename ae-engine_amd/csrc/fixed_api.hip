@@ -48,6 +48,7 @@ struct aemh_fixed {
   FxSortBufs sort{};
   uint32_t sort_cap = 0;
   uint32_t since_sort = 0;    // substeps since the P2G order was sorted
+  uint32_t sort_every = 1;    // re-sort interval in substeps (aemh_fixed_set_sort_interval)
   bool order_valid = false;   // perm is a permutation of the current particles
   double time = 0.0;
   uint64_t step_count = 0, next_id = 1;
@@ -58,9 +59,8 @@ namespace {
 using namespace q32;
 
 constexpr uint32_t kMaxParticles = 1u << 30;
-// P2G reuses its cell order this many substeps (particles move under a
-// cell per substep, so the runs of equal cells stay mostly whole)
-constexpr uint32_t kFxResortEvery = 4;
+// Longest re-sort interval aemh_fixed_set_sort_interval accepts.
+constexpr uint32_t kFxMaxSortEvery = 1u << 16;
 // Below this many particles P2G scatters in particle order with one atomic
 // per value: the cell sort's extra launches cost more than they save.
 constexpr uint32_t kFxSortMin = 8192;
@@ -379,7 +379,7 @@ aemh_status substep(aemh_fixed* s, int64_t sdt) {
   if (sorted) {
     FXCHK(ensure_sort(s, s->n));
     resort = !s->order_valid ? FX_ORDER_FROM_PLANES
-                             : (s->since_sort >= kFxResortEvery ? FX_ORDER_RESORT : FX_ORDER_KEEP);
+                             : (s->since_sort >= s->sort_every ? FX_ORDER_RESORT : FX_ORDER_KEEP);
     s->since_sort = resort != FX_ORDER_KEEP ? 1 : s->since_sort + 1;
     s->order_valid = true;
   } else {
@@ -584,6 +584,14 @@ aemh_status aemh_fixed_set_material_model(aemh_fixed* s, uint32_t model, int64_t
   if (model > AEMH_FX_SNOW) return fxfail(s, AEMH_ERR_ARG, "material model %u", model);
   s->cfg.material_model = model;
   s->cfg.model_param = model_param;
+  return AEMH_OK;
+}
+
+aemh_status aemh_fixed_set_sort_interval(aemh_fixed* s, uint32_t substeps) {
+  if (!s) return AEMH_ERR_ARG;
+  if (substeps < 1 || substeps > kFxMaxSortEvery)
+    return fxfail(s, AEMH_ERR_ARG, "sort interval %u outside [1, %u]", substeps, kFxMaxSortEvery);
+  s->sort_every = substeps;
   return AEMH_OK;
 }
 

@@ -421,6 +421,12 @@ aemh_status aemh_fixed_remove_particle(aemh_fixed* sim, uint64_t id, int* remove
 aemh_status aemh_fixed_step(aemh_fixed* sim, uint32_t n_steps);
 /* set_material_model (mpm.rs:780-782). */
 aemh_status aemh_fixed_set_material_model(aemh_fixed* sim, uint32_t model, int64_t model_param);
+/* Interval, in substeps, at which the cell-ordered P2G re-sorts its order
+ * from the live particle records (default 1: every substep; between
+ * re-sorts it reuses the last order, which stays exact but loses runs of
+ * equal cells as particles move).  Results do not depend on it, bit for
+ * bit.  Accepts 1..65536.  No reference counterpart (a tuning knob). */
+aemh_status aemh_fixed_set_sort_interval(aemh_fixed* sim, uint32_t substeps);
 /* enable_fracture / disable_fracture (mpm.rs:784-791). */
 aemh_status aemh_fixed_enable_fracture(aemh_fixed* sim, int64_t threshold);
 aemh_status aemh_fixed_disable_fracture(aemh_fixed* sim);

@@ -614,14 +614,18 @@ def test_gpu_fixed_scatter_check_paths():
 @pytest.mark.gpu
 def test_gpu_fixed_cloud_reused_order():
     """20,000 particles in random order (the benchmark cloud, scaled down)
-    for 10 steps: P2G sorts them by base cell once and reuses that order for
-    later substeps while particles cross cells, so equal cells end up in
-    several runs of a wave.  Every step bit-exact."""
+    for 10 steps with a re-sort interval of 4: P2G sorts them by base cell
+    and reuses that order for the next substeps while particles cross cells,
+    so equal cells end up in several runs of a wave.  Every step bit-exact."""
     import aemh
     from aemh import workloads as W
     sc = W.fixed_cloud(20000, 32)
     print("fixed cloud 20k, reused P2G order", flush=True)
     g, o = _gpu_pair(sc.config)
+    for bad in (0, 65537):
+        with pytest.raises(aemh.AemhError):
+            g.set_sort_interval(bad)
+    g.set_sort_interval(4)
     zero = np.zeros_like(sc.pos)
     g.add_particles(sc.pos, zero, sc.mass)
     o.add_particles(sc.pos, zero, sc.mass)

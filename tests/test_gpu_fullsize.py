@@ -87,7 +87,7 @@ def test_fixed_point_full_size_order_invariant(n, res, steps):
     saturating Q32.32 sums; while no cell can clip (the scatter bound, which
     the run reports), they are plain integer sums, so permuting the particles
     before they are added leaves every raw field of every particle unchanged.
-    Both runs cross a re-sort of the cell order (every 4th substep)."""
+    The cell order of P2G is re-sorted from the live records every substep."""
     sc = W.fixed_cloud(n, res)
     perm = np.random.default_rng(7).permutation(n)
     a, sa = fixed_run(sc, steps)
