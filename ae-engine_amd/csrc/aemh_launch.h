@@ -12,10 +12,9 @@ namespace aemh {
 
 // Per-kernel timing of profiled steps.  While a stage is timed, its launches
 // hand the stage's events to hipExtLaunchKernel: the start comes from the
-// first kernel's dispatch and the stop from the last one's, with no marker
-// packets between kernels.  (Event records between the kernels slowed the
-// kernel after each of them: P2G of the 1M snow workload took 178 us against
-// 132 us in graph replay and in plain eager launches.)
+// first kernel's dispatch and the stop from the last one's, so no marker
+// packets sit between the kernels and the timed sequence is the one the
+// graph replays.
 struct StageEvents {
   hipEvent_t start = nullptr, stop = nullptr;
   bool launched = false;
