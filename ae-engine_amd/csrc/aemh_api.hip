@@ -1245,10 +1245,32 @@ aemh_status aemh_set_scenes(aemh_ctx* c, const aemh_scene* scenes, uint32_t n) {
   return AEMH_OK;
 }
 
+namespace {
+// Before any batched launch: calls made since aemh_set_scenes may have
+// replaced the particles (upload, J_p upload, storage change: the working
+// set is stale) or changed the configuration the scene table and the grid
+// buffers were built from (accumulation mode, gravity, damping, walls).
+// The working set is gathered again and the table rebuilt from the scenes
+// as given; a table that no longer tiles the new working set is an error.
+// An unchanged table is neither re-uploaded nor re-allocated.
+aemh_status revalidate_batch(aemh_ctx* c) {
+  if (!c->work_valid) {
+    aemh_status s = ensure_work(c);
+    if (s != AEMH_OK) return s;
+  }
+  aemh_status s = set_scene_table(c, c->scenes_h);
+  if (s != AEMH_OK) return s;
+  c->batch_mode = true;
+  return AEMH_OK;
+}
+}  // namespace
+
 aemh_status aemh_batch_step(aemh_ctx* c, uint32_t n_substeps) {
   if (!c) return AEMH_ERR_ARG;
   if (!c->batch_mode) return fail(c, AEMH_ERR_STATE, "aemh_set_scenes first");
   hipSetDevice(c->device);
+  aemh_status r = revalidate_batch(c);
+  if (r != AEMH_OK) return r;
   if (!c->tuned) {
     aemh_status s = aemh_autotune(c);
     if (s != AEMH_OK) return s;
@@ -1267,8 +1289,8 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   if (!c) return AEMH_ERR_ARG;
   if (!c->mats_d) return fail(c, AEMH_ERR_STATE, "materials not set");
   hipSetDevice(c->device);
-  aemh_status s = AEMH_OK;
-  if (!c->batch_mode) s = prepare_single_scene(c, c->scd_h.empty() ? 1e-4f : c->scd_h[0].dt, 0);
+  aemh_status s = c->batch_mode ? revalidate_batch(c)
+                                : prepare_single_scene(c, c->scd_h.empty() ? 1e-4f : c->scd_h[0].dt, 0);
   if (s != AEMH_OK) return s;
   if (c->n_work == 0) return AEMH_OK;
   const bool prof = c->prof;

@@ -318,8 +318,12 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
     const float vv[3] = {q.vx, q.vy, q.vz};
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-      bp = fmaxf(bp, fabsf(vv[r]) + 2.0f * S.dx * (fabsf(q.c[3 * r]) + fabsf(q.c[3 * r + 1]) + fabsf(q.c[3 * r + 2])));
-      bf = fmaxf(bf, S.inv_dx * (fabsf(q.sv[3 * r]) + fabsf(q.sv[3 * r + 1]) + fabsf(q.sv[3 * r + 2])));
+      const float tp = fabsf(vv[r]) + 2.0f * S.dx * (fabsf(q.c[3 * r]) + fabsf(q.c[3 * r + 1]) + fabsf(q.c[3 * r + 2]));
+      const float tf = S.inv_dx * (fabsf(q.sv[3 * r]) + fabsf(q.sv[3 * r + 1]) + fabsf(q.sv[3 * r + 2]));
+      // fmaxf would drop a NaN term: test each term itself (false for NaN)
+      if (!(tp <= 3.0e38f) || !(tf <= 3.0e38f)) finite_in = false;
+      bp = fmaxf(bp, tp);
+      bf = fmaxf(bf, tf);
     }
     const float b[3] = {2.0f * q.mass, 2.0f * q.mass * bp, 2.0f * bf};
 #pragma unroll
@@ -947,8 +951,9 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
     // particle's base node.  Nodes past the grid's far faces get weight 0
     // (an axis weight, so w0 = 0 < 1e-10 skips them as the in-grid test
     // does); nodes with m < 1e-12 were staged with v = 0.  Only the weight
-    // test is left per node: u = w g with w = 0 for a skipped node adds +-0
-    // to the sums, the same bits as skipping it while g is finite.
+    // test is left per node: a skipped node's u is a selected +0 (not a
+    // product with a zero weight, which a non-finite grid velocity would
+    // turn into NaN), the same bits as the reference skipping it.
     float wxm[3], wym[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -978,8 +983,8 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
         for (int di = 0; di < 3; di++) {
           const float w0 = wxy[dj][di] * wzk;
           const float4 g = tk[di + kTileSide * dj];
-          const float w = !(w0 < 1e-10f) ? w0 : 0.0f;
-          const float u[3] = {w * g.x, w * g.y, w * g.z};
+          const bool keep = !(w0 < 1e-10f);
+          const float u[3] = {keep ? w0 * g.x : 0.0f, keep ? w0 * g.y : 0.0f, keep ? w0 * g.z : 0.0f};
 #pragma unroll
           for (int r = 0; r < 3; r++) {
             sx[di][r] += u[r];

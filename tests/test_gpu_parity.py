@@ -483,3 +483,41 @@ def test_conservation_f64_matches_numpy():
     assert abs(out[0] - m.sum()) <= 1e-12 * m.sum()
     assert np.allclose(out[1:4], (m[:, None] * v).sum(0), rtol=1e-9, atol=1e-18)
     assert abs(out[4] - 0.5 * (m * (v * v).sum(1)).sum()) <= 1e-9 * abs(out[4]) + 1e-20
+
+
+def test_fp32_nan_velocity_next_to_zero_weight_node():
+    """A particle with a NaN velocity makes its nodes' momenta NaN.  A second
+    particle whose stencil reaches one of those nodes with a weight below
+    1e-10 skips it (mpm_solver.rs:313-314), so its velocity and C stay finite
+    and equal the oracle's; a zero weight multiplied into the NaN would have
+    zeroed them through the non-finite guard (Q7)."""
+    dx = np.float32(1.0 / 32.0)
+    b, c = 10, 12
+    # B: x-axis fraction 0.5 + 1e-5, so its third x node (b + 2) has weight
+    # 0.5 (1e-5)^2 times the y/z weights, far below 1e-10.  A: base b + 2.
+    xb = (b + 0.5 + 0.5 + 1e-5) * dx
+    xa = (b + 2 + 0.5 + 0.2) * dx
+    yz = (c + 0.5 + 0.3) * dx
+    pos = np.array([[xa, yz, yz], [xb, yz, yz], [xb - 0.3 * dx, yz, yz + 0.2 * dx]], np.float32)
+    vel = np.array([[np.nan, 0.0, 0.0], [0.1, -0.2, 0.05], [0.0, 0.1, 0.0]], np.float32)
+    mass = np.full(3, 1e-3, np.float32)
+    mats = [aemh.Material(1e3, 0.2, 0, 0, 1.0, 0, 0, aemh.NEWTONIAN_FLUID)]
+    eng = aemh.Engine()
+    cfg = aemh.default_config()
+    cfg.accum = aemh.ACCUM_F32
+    eng.set_config(cfg)
+    eng.set_grid(32, 32, 32, float(dx))
+    eng.set_materials(mats)
+    eng.upload(pos, vel=vel, mass=mass)
+    eng.step(1e-4)
+    d = eng.download()
+    eng.close()
+    st = R.State(pos, vel=vel, mass=mass)
+    p, m, e = aemh.material_arrays(mats)
+    R.step(R.make_config(32, 32, 32, float(dx)), R.Materials(p, m, e), st, 1e-4)
+    # the lone NaN particle: v and C zeroed by Q7 on both sides
+    assert np.all(d["vel"][0] == 0) and np.all(st.vel[0] == 0)
+    for k in (1, 2):
+        assert np.all(np.isfinite(d["vel"][k])) and np.any(d["vel"][k] != 0)
+        assert np.allclose(d["vel"][k], st.vel[k], rtol=1e-5, atol=1e-9), (k, d["vel"][k], st.vel[k])
+        assert np.allclose(d["C"][k], st.C[k], rtol=1e-4, atol=1e-6), k
