@@ -148,6 +148,7 @@ EXPORTS = [
     "aemh_step_velocity_only", "aemh_set_scenes", "aemh_batch_step", "aemh_autotune",
     "aemh_get_launch_shape", "aemh_set_launch_shape",
     "aemh_download_grid", "aemh_download_grid_q32", "aemh_conservation_f64",
+    "aemh_conservation_f64_noalloc",
     "aemh_last_substep_info", "aemh_profile_enable", "aemh_profile_get", "aemh_profile_reset",
     "aemh_scene_uniform_box",
     "aemh_mpss_reserve", "aemh_mpss_spawn", "aemh_mpss_kill", "aemh_mpss_compact",
@@ -206,6 +207,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.aemh_download_grid.argtypes = [P, u32, fp, fp, fp, fp]
     lib.aemh_download_grid_q32.argtypes = [P, u32, C.POINTER(C.c_int64)]
     lib.aemh_conservation_f64.argtypes = [P, C.POINTER(C.c_double)]
+    lib.aemh_conservation_f64_noalloc.argtypes = [P, C.POINTER(C.c_double)]
     lib.aemh_last_substep_info.argtypes = [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     lib.aemh_profile_enable.argtypes = [P, C.c_int]
     lib.aemh_profile_get.argtypes = [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
@@ -461,6 +463,13 @@ class Engine:
     def conservation_f64(self):
         out = (C.c_double * 5)()
         self._check(self._lib.aemh_conservation_f64(self._h, out))
+        return np.array(out[:], np.float64)
+
+    def conservation_f64_noalloc(self):
+        """conservation_f64 through the entry point that allocates nothing
+        (its partial sums use a static device buffer)."""
+        out = (C.c_double * 5)()
+        self._check(self._lib.aemh_conservation_f64_noalloc(self._h, out))
         return np.array(out[:], np.float64)
 
     def last_substep_info(self):

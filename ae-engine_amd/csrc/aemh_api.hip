@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -44,6 +45,16 @@ constexpr size_t kInfinityCacheBytes = (size_t)256 << 20;
 // Working-set slots are addressed with 32-bit byte offsets in the kernels.
 constexpr uint32_t kMaxWork = 1u << 30;
 
+// Device memory of aemh_conservation_f64_noalloc: the 5 sums followed by the
+// 5 partial sums of each reduction workgroup (kConsScratch doubles, 20.5 KB),
+// one copy per device in the loaded code object, held until the process
+// ends.  In host code the name is not a device address: the kernels get the
+// address hipGetSymbolAddress returns for the current device (kept per
+// device in g_scratch by scratch_for_device, called by aemh_create).
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+double* g_scratch[kMaxDevices] = {};
+
 struct ParticleStore {
   uint32_t cap = 0;
   int storage = 0;
@@ -58,8 +69,13 @@ struct ProfRec {
 
 }  // namespace
 
+static __device__ __align__(256) double scratch[aemh::kConsScratch];
+
 struct aemh_ctx {
   int device = 0;
+  // this device's address of `scratch` (nullptr if it could not be had)
+  double* cons_scratch = nullptr;
+  std::string scratch_err;
   hipStream_t stream = nullptr;
   std::string err;
   aemh_config cfg{};
@@ -153,6 +169,21 @@ struct aemh_ctx {
 };
 
 namespace {
+
+// This device's address of `scratch`, looked up once per device with the
+// device current (each device holds its own copy at its own address).
+hipError_t scratch_for_device(int device, double** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch[device];
+  return hipSuccess;
+}
 
 aemh_status fail(aemh_ctx* c, aemh_status s, const char* fmt, ...) {
   char buf[512];
@@ -983,6 +1014,13 @@ aemh_status aemh_create(aemh_ctx** out, int device) {
     return AEMH_ERR_HIP;
   }
   *c->runs_h = 0;
+  // Only aemh_conservation_f64_noalloc needs it: a failure is kept and
+  // reported by that entry point.
+  const hipError_t se = scratch_for_device(device, &c->cons_scratch);
+  if (se != hipSuccess) {
+    c->cons_scratch = nullptr;
+    c->scratch_err = hipGetErrorName(se);
+  }
   *out = c;
   return AEMH_OK;
 }
@@ -1458,12 +1496,39 @@ aemh_status aemh_conservation_f64(aemh_ctx* c, double* out5) {
   aemh_status s = ensure_work(c);
   if (s != AEMH_OK) return s;
   double* tmp = nullptr;
-  HIPTRY(c, hipMalloc(&tmp, (5 + 5 * 512) * sizeof(double)));
+  HIPTRY(c, hipMalloc(&tmp, kConsScratch * sizeof(double)));
   launch_conservation(c->stream, c->work[c->cur].pb, c->n_work, tmp);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(out5, tmp, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   hipFree(tmp);
+  if (e != hipSuccess) return fail(c, AEMH_ERR_HIP, "reduction: %s", hipGetErrorName(e));
+  return AEMH_OK;
+}
+
+// The same sums with no device allocation: the partials and the result go
+// to this device's `scratch`, which holds what any working set needs
+// (kConsScratch doubles, independent of the particle count), so no input
+// exceeds it.  Every value there is written before it is read, so nothing
+// is cleared first.  All calls on one device share that memory: calls that
+// overlap on one device (two host threads, or two contexts of the device)
+// race.  A working set that is not ready would have to be gathered, which
+// allocates, so that case is refused.
+aemh_status aemh_conservation_f64_noalloc(aemh_ctx* c, double* out5) {
+  if (!c || !out5) return fail(c, AEMH_ERR_ARG, "null argument");
+  static_assert(kConsScratch == 5 + 5 * kConsWgs, "scratch holds the sums and every partial");
+  hipSetDevice(c->device);
+  if (!c->cons_scratch)
+    return fail(c, AEMH_ERR_HIP, "no device address for the reduction scratch: %s",
+                c->scratch_err.c_str());
+  if (!(c->work_valid && (c->idx_explicit || c->work_default || !c->mpss)))
+    return fail(c, AEMH_ERR_STATE,
+                "no current working set: step or set the indices first (gathering one allocates)");
+  launch_conservation(c->stream, c->work[c->cur].pb, c->n_work, c->cons_scratch);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out5, c->cons_scratch, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return fail(c, AEMH_ERR_HIP, "reduction: %s", hipGetErrorName(e));
   return AEMH_OK;
 }

@@ -65,6 +65,10 @@ void launch_grid_readback(hipStream_t st, const SceneDev* sc_d, int scene, Scene
                           SortDev s, GridDev g, int q32, float* m, float* p3, float* f3,
                           float* v3, long long* q7, int upd_wgs);
 // FP64 sums over the working set: mass, momentum xyz, kinetic energy.
+// out5 needs room for kConsScratch doubles: the 5 sums, then the 5 partial
+// sums of each of the kConsWgs reduction workgroups (every one is written).
+constexpr int kConsWgs = 512;
+constexpr int kConsScratch = 5 + 5 * kConsWgs;
 void launch_conservation(hipStream_t st, PBuf cur, uint32_t n, double* out5);
 
 // ---- k_mpss.hip: resident MpssBuffer lifecycle ------------------------------

@@ -342,7 +342,6 @@ __global__ __launch_bounds__(256) void k_grid_readback(const SceneDev* __restric
 
 // FP64 partial sums per workgroup, then one workgroup adds the partials in a
 // fixed order: deterministic reductions.
-constexpr int kRedWgs = 512;
 __global__ __launch_bounds__(256) void k_conservation_partial(PBuf cur, uint32_t n, double* part) {
   __shared__ double sh[5][256];
   double acc[5] = {0, 0, 0, 0, 0};
@@ -415,9 +414,8 @@ void launch_grid_readback(hipStream_t st, const SceneDev* sc_d, int scene, Scene
   k_grid_readback<<<upd_wgs, 256, 0, st>>>(sc_d, scene, s, g, q32, m, p3, f3, v3, q7);
 }
 void launch_conservation(hipStream_t st, PBuf cur, uint32_t n, double* out5) {
-  // out5 must have room for 5 + 5 * kRedWgs doubles (partials follow the result).
-  k_conservation_partial<<<kRedWgs, 256, 0, st>>>(cur, n, out5 + 5);
-  k_conservation_final<<<1, 64, 0, st>>>(out5 + 5, kRedWgs, out5);
+  k_conservation_partial<<<kConsWgs, 256, 0, st>>>(cur, n, out5 + 5);
+  k_conservation_final<<<1, 64, 0, st>>>(out5 + 5, kConsWgs, out5);
 }
 
 }  // namespace aemh

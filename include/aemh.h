@@ -279,6 +279,16 @@ aemh_status aemh_download_grid_q32(aemh_ctx* ctx, uint32_t scene, int64_t* q7);
 /* FP64 reductions over the active set: total mass, momentum[3], kinetic
  * energy (ae_physics/src/mpm.rs:752-771 diagnostics, in FP64). out[5]. */
 aemh_status aemh_conservation_f64(aemh_ctx* ctx, double* out5);
+/* The same FP64 sums, bit for bit, with no device allocation: no counterpart
+ * in the reference.  For callers that cannot allocate device memory between
+ * steps (tight stepping loops that take these diagnostics every few steps,
+ * harnesses that forbid hipMalloc/hipFree).  The reduction's partial sums
+ * live in a fixed static device buffer of 5 + 5 x 512 doubles (20.5 KB per
+ * device, held until the process ends), enough for any particle count.
+ * Needs a current working set (after a step or aemh_set_indices; otherwise
+ * AEMH_ERR_STATE, since gathering one allocates).  All calls on one device
+ * share the buffer: calls that overlap on one device race. */
+aemh_status aemh_conservation_f64_noalloc(aemh_ctx* ctx, double* out5);
 
 /* Work of the last substep: active 4^3 node blocks (grid nodes touched =
  * 64 x blocks) and particles advanced. */
