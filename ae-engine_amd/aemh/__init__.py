@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(_HERE, "libaemh.so")
 OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_STATE, ERR_OVERFLOW, ERR_NO_DEVICE = range(7)
 # ConstitutiveModel order (ae_particle/src/constitutive.rs:11-27) + SNOW
 NEO_HOOKEAN, DRUCKER_PRAGER, VON_MISES, NEWTONIAN_FLUID, COROTATED, FIXED_COROTATED, RIGID, SNOW = range(8)
-ACCUM_F32, ACCUM_Q32_32 = 0, 1
+ACCUM_F32, ACCUM_Q32_32, ACCUM_F32_SERIAL = 0, 1, 2
 CLAMP_AUTO, CLAMP_SERIAL, CLAMP_WINDOW = 0, 1, 2
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -166,6 +167,11 @@ struct aemh_ctx {
   std::vector<hipEvent_t> ev_pool;
   // pinned readback word for overflow checks
   uint32_t* flag_h = nullptr;
+  // reference-order FP32 mode (AEMH_ACCUM_F32_SERIAL): entry buffers for
+  // ser_cap entries and the device copy of the host logf table
+  SerialDev ser{};
+  size_t ser_cap = 0;
+  float* lntab_d = nullptr;
 };
 
 namespace {
@@ -631,6 +637,84 @@ aemh_status mpss_ready(aemh_ctx* c) {
   return AEMH_OK;
 }
 
+// logf of the host C library for every float in [0.1, 10] (kLnTab*), built
+// once per process: the values the reference's f32::ln gets from the
+// platform libm (constitutive.rs:384).
+const std::vector<float>& host_logf_table() {
+  static std::once_flag once;
+  static std::vector<float> tab;
+  std::call_once(once, [] {
+    tab.resize(kLnTabSize);
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> ts;
+    for (unsigned t = 0; t < nt; t++)
+      ts.emplace_back([t, nt] {
+        for (uint32_t i = t; i < kLnTabSize; i += nt) {
+          const uint32_t b = kLnTabLo + i;
+          float x;
+          memcpy(&x, &b, 4);
+          tab[i] = logf(x);
+        }
+      });
+    for (auto& th : ts) th.join();
+  });
+  return tab;
+}
+
+int bit_length(uint64_t v) {
+  int b = 0;
+  while (v) {
+    b++;
+    v >>= 1;
+  }
+  return b;
+}
+
+// Buffers of the reference-order FP32 P2G for the current working set and
+// grid: 27 entries per particle (keys and values double-buffered for the
+// sort, 8 contribution floats each, 56 B per entry), the radix sort's
+// temporary storage, and the logf table (uploaded once per context).
+aemh_status ensure_serial(aemh_ctx* c) {
+  SerialDev& sd = c->ser;
+  if (!c->lntab_d) {
+    const std::vector<float>& tab = host_logf_table();
+    HIPTRY(c, hipMalloc(&c->lntab_d, (size_t)kLnTabSize * 4));
+    HIPTRY(c, hipMemcpy(c->lntab_d, tab.data(), (size_t)kLnTabSize * 4, hipMemcpyHostToDevice));
+  }
+  sd.lntab = c->lntab_d;
+  const size_t m = (size_t)(c->n_work ? c->n_work : 1) * 27;
+  if (m >= (1ull << 31)) return fail(c, AEMH_ERR_ARG, "serial-order P2G: too many particles");
+  sd.id_bits = std::max(1, bit_length(c->n_work ? c->n_work - 1 : 0));
+  sd.end_bit = sd.id_bits + bit_length(c->grid.nodes);
+  if (m > c->ser_cap) {
+    for (void* p : {(void*)sd.key[0], (void*)sd.key[1], (void*)sd.val[0], (void*)sd.val[1],
+                    (void*)sd.contrib, sd.tmp})
+      if (p) hipFree(p);
+    sd.key[0] = sd.key[1] = nullptr;
+    sd.val[0] = sd.val[1] = nullptr;
+    sd.contrib = nullptr;
+    sd.tmp = nullptr;
+    sd.tmp_bytes = 0;
+    c->ser_cap = 0;
+    for (int k = 0; k < 2; k++) {
+      HIPTRY(c, hipMalloc(&sd.key[k], m * 8));
+      HIPTRY(c, hipMalloc(&sd.val[k], m * 4));
+    }
+    HIPTRY(c, hipMalloc(&sd.contrib, m * 32));
+    c->ser_cap = m;
+  }
+  // the temporary size is largest with all 64 key bits
+  const size_t tb = sort_pairs_u64_temp_bytes((uint32_t)c->ser_cap, 64);
+  if (tb > sd.tmp_bytes) {
+    if (sd.tmp) hipFree(sd.tmp);
+    sd.tmp = nullptr;
+    sd.tmp_bytes = 0;
+    HIPTRY(c, hipMalloc(&sd.tmp, tb));
+    sd.tmp_bytes = tb;
+  }
+  return AEMH_OK;
+}
+
 // Launch one substep on the context stream (eager or under capture).
 // One substep on work[cur].  A re-sort step writes G2P's result to
 // work[cur ^ 1] in block order; any other step updates work[cur] in place.
@@ -645,7 +729,19 @@ void enqueue_substep(aemh_ctx* c, int cur, bool resort) {
   timed_k(c, "count", [&] { launch_count(st, a, c->scd_d, ns, n, c->sort, c->nb_total); });
   timed_k(c, "scan", [&] { launch_scan(st, c->scd_d, ns, c->sort, c->nb_total); });
   timed_k(c, "scatter", [&] { launch_scatter(st, n, c->nb_total, c->sort, c->grid, q32); });
-  if (q32) {
+  if (c->cfg.accum == AEMH_ACCUM_F32_SERIAL) {
+    // FP32 sums in list order, the FP32 grid update (atomic-flush layout:
+    // no P2G partials), the reference-order G2P of the deterministic build
+    LaunchShape ls = c->shape;
+    ls.p2g_partials = 0;
+    timed_k(c, "p2g", [&] {
+      det::launch_p2g_serial(st, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->ser, c->grid, storage);
+    });
+    timed_k(c, "grid_update", [&] { fast::launch_grid_update(st, ls, c->scd_d, ns, c->sort, c->grid, n, c->nb_total); });
+    timed_k(c, "g2p", [&] {
+      det::launch_g2p(st, ls, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, inplace);
+    });
+  } else if (q32) {
     timed_k(c, "p2g", [&] {
       det::launch_p2g(st, c->shape, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
     });
@@ -697,14 +793,21 @@ aemh_status check_overflow(aemh_ctx* c) {
 aemh_status run_substeps(aemh_ctx* c, uint32_t k, bool allow_graph = true) {
   if (!c->mats_d) return fail(c, AEMH_ERR_STATE, "materials not set");
   if (c->n_work == 0) return AEMH_OK;
-  if (c->shape.p2g_partials) {
+  const bool serial = c->cfg.accum == AEMH_ACCUM_F32_SERIAL;
+  if (c->shape.p2g_partials && !serial) {
     aemh_status s = ensure_partials(c);
+    if (s != AEMH_OK) return s;
+  }
+  if (serial) {
+    aemh_status s = ensure_serial(c);
     if (s != AEMH_OK) return s;
   }
   // Steps replay the captured substep graph.  With profiling on they launch
   // eagerly with an event pair around each kernel section: event-record
-  // nodes inside a graph cost the replay more than eager launches do.
-  const bool graph = allow_graph && c->use_graph && !c->prof;
+  // nodes inside a graph cost the replay more than eager launches do.  The
+  // reference-order mode launches eagerly: its library radix sort is not
+  // captured.
+  const bool graph = allow_graph && c->use_graph && !c->prof && !serial;
   bool counted_sorted = false;  // the last substep's count saw a re-sorted set
   for (uint32_t i = 0; i < k; i++) {
     // Re-sort on the first substep of a new working set, when the block
@@ -853,7 +956,9 @@ aemh_status ensure_tier(aemh_ctx* c) {
   for (void* p : {(void*)c->tier_list[0], (void*)c->tier_list[1], (void*)c->tier_list[2],
                   (void*)c->tier_cls, (void*)c->tier_dist, (void*)c->tier_keys[0],
                   (void*)c->tier_keys[1], (void*)c->tier_vals, (void*)c->tier_bcnt,
-                  c->tier_sort_tmp})
+                  c->tier_sort_tmp, (void*)c->ser.key[0], (void*)c->ser.key[1],
+                  (void*)c->ser.val[0], (void*)c->ser.val[1], (void*)c->ser.contrib, c->ser.tmp,
+                  (void*)c->lntab_d})
     if (p) hipFree(p);
   c->tier_cap = 0;
   for (int t = 0; t < 3; t++) c->tier_n[t] = 0;
@@ -1062,7 +1167,7 @@ const char* aemh_last_error(const aemh_ctx* c) { return c ? c->err.c_str() : "nu
 aemh_status aemh_set_config(aemh_ctx* c, const aemh_config* cfg) {
   if (!c || !cfg) return fail(c, AEMH_ERR_ARG, "null argument");
   if (cfg->substeps == 0) return fail(c, AEMH_ERR_ARG, "substeps must be >= 1");
-  if (cfg->accum > AEMH_ACCUM_Q32_32 || cfg->clamp_mode > AEMH_CLAMP_WINDOW ||
+  if (cfg->accum > AEMH_ACCUM_F32_SERIAL || cfg->clamp_mode > AEMH_CLAMP_WINDOW ||
       cfg->storage > AEMH_STORE_F16)
     return fail(c, AEMH_ERR_ARG, "unknown mode value");
   hipSetDevice(c->device);

@@ -120,6 +120,27 @@ struct SortDev {
                         // holding its first particle (>= tiles: no tile)
 };
 
+// The float range of the host logf table of the reference-order FP32 mode:
+// [0.1, 10], where G2P clamps J (mpm_solver.rs:1040-1042).  Positive floats
+// order as their bit patterns, so entry i is logf of the float with bits
+// kLnTabLo + i.
+constexpr uint32_t kLnTabLo = 0x3DCCCCCDu;  // bits of 0.1f
+constexpr uint32_t kLnTabHi = 0x41200000u;  // bits of 10.0f
+constexpr uint32_t kLnTabSize = kLnTabHi - kLnTabLo + 1u;
+
+// Reference-order FP32 P2G (AEMH_ACCUM_F32_SERIAL): every visited stencil
+// node of every working slot is an entry, sorted by (node, list position)
+// so each node's sums run in the reference's particle order.
+struct SerialDev {
+  unsigned long long* key[2];  // [27 N] (node << id_bits) | list position; unused: nodes << id_bits
+  uint32_t* val[2];            // [27 N] entry slot 27 j + k
+  float* contrib;              // [27 N][8] m, p0..2, f0..2 (pad) of each entry
+  const float* lntab;          // host logf of every float in [0.1, 10]
+  void* tmp;                   // radix-sort temporary storage
+  size_t tmp_bytes;
+  int id_bits, end_bit;        // bits of the list position; key bits to sort
+};
+
 constexpr int kScanTile = 1024;
 constexpr int kRunLanes = 64;
 constexpr int kRunStride = 16;

@@ -102,6 +102,11 @@ void launch_select(hipStream_t st, const uint8_t* cls, const uint32_t* src, cons
                    uint32_t n, uint32_t* bcnt, uint32_t* boff, uint32_t* tot, uint32_t* out0,
                    uint32_t* out1, uint32_t* out2, uint32_t* key0);
 size_t sort_pairs_temp_bytes(uint32_t n);
+// Radix sort of 64-bit keys (bits [0, end_bit)) with 32-bit values.
+size_t sort_pairs_u64_temp_bytes(uint32_t n, int end_bit);
+void launch_sort_pairs_u64(hipStream_t st, void* tmp, size_t tmp_bytes,
+                           const unsigned long long* kin, unsigned long long* kout,
+                           const uint32_t* vin, uint32_t* vout, uint32_t n, int end_bit);
 hipError_t launch_sort_pairs(hipStream_t st, void* tmp, size_t tmp_bytes, const uint32_t* kin,
                              uint32_t* kout, const uint32_t* vin, uint32_t* vout, uint32_t n);
 void launch_far_update(hipStream_t st, PBuf P, const uint32_t* far, uint32_t n, float far_dt);
@@ -122,5 +127,11 @@ void launch_gather_pos(hipStream_t st, PBuf P, const uint32_t* list, uint32_t n,
 AEMH_DECLARE_MODE(fast)
 AEMH_DECLARE_MODE(det)
 #undef AEMH_DECLARE_MODE
+namespace det {
+// Reference-order FP32 P2G: the grid's FP32 sums (accf) of the active
+// blocks, each node's contributions added in list order.
+void launch_p2g_serial(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, const MatDev* mats,
+                       int nmat, uint32_t n, SerialDev sd, GridDev g, int storage);
+}
 
 }  // namespace aemh

@@ -23,6 +23,26 @@ __device__ __forceinline__ float m_ln(float x) { return logf(x); }
 __device__ __forceinline__ float m_exp(float x) { return expf(x); }
 #endif
 
+// The natural logarithm the Neo-Hookean stress takes of J: this build's
+// m_ln, unless a caller passes another (the reference-order FP32 mode passes
+// a table of the host C library's logf).
+struct ModeLn {
+  __device__ __forceinline__ float operator()(float x) const { return m_ln(x); }
+};
+
+// logf of the host C library (the function the reference's f32::ln calls,
+// constitutive.rs:384), tabulated by the host for every float in the range
+// G2P clamps J to, [0.1, 10] (mpm_solver.rs:1040-1042); other arguments (an
+// uploaded J outside it) take the portable, correctly rounded det_logf
+// (kLnTab* in aemh_common.h).
+struct TableLn {
+  const float* tab;
+  __device__ __forceinline__ float operator()(float x) const {
+    const uint32_t b = __float_as_uint(x) - kLnTabLo;  // positive floats order as their bits
+    return b < kLnTabSize ? tab[b] : det_logf(x);
+  }
+};
+
 template <typename T>
 __device__ __forceinline__ void identity3(T* m) {
 #pragma unroll
@@ -94,12 +114,13 @@ __device__ __noinline__ void polar_decomp(const float* F, float* r, float* s) {
 }
 
 // neo_hookean, constitutive.rs:375-393, with mu/lambda as arguments.
+template <class Ln = ModeLn>
 __device__ __forceinline__ void neo_hookean(const float* F, float J, float mu, float lambda,
-                                            float* st) {
+                                            float* st, Ln ln = Ln()) {
   float ft[9], b[9];
   transpose3(F, ft);
   mat_mul(F, ft, b);
-  float p_term = (J > 0.0f) ? lambda * m_ln(J) / J : 0.0f;
+  float p_term = (J > 0.0f) ? lambda * ln(J) / J : 0.0f;
   float scale = mu / fmaxf(J, 1e-10f);
 #pragma unroll
   for (int i = 0; i < 9; i++) {
@@ -189,8 +210,9 @@ __device__ __noinline__ Mat3 corotated_stress(const MatDev M, const Mat3 Fm, flo
 // compute_stress dispatch, constitutive.rs:234-249, plus the snow model
 // (Stomakhin 2013 hardening h = exp(xi (1 - J_p)) clamped to [0.1, 5] on
 // mu and lambda of the Neo-Hookean stress; no reference counterpart).
+template <class Ln = ModeLn>
 __device__ __forceinline__ void compute_stress(const MatDev& M, const float* F, float J, float Jp,
-                                               float* st) {
+                                               float* st, Ln ln = Ln()) {
   switch (M.model) {
     case kModelRigid:
       identity3(st);
@@ -227,11 +249,11 @@ __device__ __forceinline__ void compute_stress(const MatDev& M, const float* F, 
     case kModelSnow: {
       float h = m_exp(M.xi * (1.0f - Jp));
       h = fminf(fmaxf(h, 0.1f), 5.0f);
-      neo_hookean(F, J, M.mu * h, M.lambda * h, st);
+      neo_hookean(F, J, M.mu * h, M.lambda * h, st, ln);
       break;
     }
     default:
-      neo_hookean(F, J, M.mu, M.lambda, st);
+      neo_hookean(F, J, M.mu, M.lambda, st, ln);
       break;
   }
 }

@@ -222,6 +222,22 @@ hipError_t launch_sort_pairs(hipStream_t st, void* tmp, size_t tmp_bytes, const 
                                             st);
 }
 
+size_t sort_pairs_u64_temp_bytes(uint32_t n, int end_bit) {
+  size_t bytes = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const unsigned long long*)nullptr,
+                                           (unsigned long long*)nullptr, (const uint32_t*)nullptr,
+                                           (uint32_t*)nullptr, (int)n, 0, end_bit);
+  return bytes;
+}
+
+void launch_sort_pairs_u64(hipStream_t st, void* tmp, size_t tmp_bytes,
+                           const unsigned long long* kin, unsigned long long* kout,
+                           const uint32_t* vin, uint32_t* vout, uint32_t n, int end_bit) {
+  if (n)
+    (void)hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kin, kout, vin, vout, (int)n, 0,
+                                             end_bit, st);
+}
+
 void launch_far_update(hipStream_t st, PBuf P, const uint32_t* far, uint32_t n, float far_dt) {
   if (n) k_far_update<<<grid_of(n), 256, 0, st>>>(P, far, n, far_dt);
 }

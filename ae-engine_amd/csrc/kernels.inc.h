@@ -95,6 +95,50 @@ __device__ __forceinline__ void p2g_nodes(const int n0, const int n1, const int 
       }
 }
 
+// The P2G inputs of working slot p: position, velocity, mass, the B-spline
+// weights and derivatives of its stencil and, in full MPM, C and the
+// volume-scaled stress of its material (material_table.get(i), else entry
+// 0 as NeoHookean, mpm_solver.rs:839-849; the corrected F is discarded).
+// k_p2g writes the same loads out in its body: through this function the
+// compiler schedules them differently and k_p2g takes 16 B more scratch and
+// 4 more VGPRs per lane.
+template <int STORE, class Ln = ModeLn>
+__device__ __forceinline__ void p2g_inputs(const PBuf P, uint32_t p, const SceneDev& S,
+                                           const Stencil& st, const MatDev* __restrict__ mats,
+                                           int nmat, bool full, P2GPart& q, Ln ln = Ln()) {
+  const float inv_dx = S.inv_dx;
+  q.px = P.x[0][p];
+  q.py = P.x[1][p];
+  q.pz = P.x[2][p];
+  q.vx = P.v[0][p];
+  q.vy = P.v[1][p];
+  q.vz = P.v[2][p];
+  q.mass = P.mass[p];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    q.wx[a] = b_spline((float)a - st.fx[0]);
+    q.wy[a] = b_spline((float)a - st.fx[1]);
+    q.wz[a] = b_spline((float)a - st.fx[2]);
+    q.dwx[a] = b_spline_deriv((float)a - st.fx[0]) * inv_dx;
+    q.dwy[a] = b_spline_deriv((float)a - st.fx[1]) * inv_dx;
+    q.dwz[a] = b_spline_deriv((float)a - st.fx[2]) * inv_dx;
+  }
+  if (full) {
+    float F[9];
+    load_cf(P, p, q.c, F, STORE);
+    const uint32_t mi = P.mat[p];
+    const MatDev M = mats[mi < (uint32_t)nmat ? mi : (uint32_t)nmat];
+    float stress[9];
+    compute_stress(M, F, P.J[p], P.Jp[p], stress, ln);
+    const float vol = q.mass / M.density;
+#pragma unroll
+    for (int k = 0; k < 9; k++) q.sv[k] = vol * stress[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; k++) q.c[k] = q.sv[k] = 0.0f;
+  }
+}
+
 // trunc(|x|) as an integer for |x| < 2^63, exactly: hi = floor(|x| / 2^32)
 // and lo = |x| - hi 2^32 are both exact in float (x has 24 significant bits).
 __device__ __forceinline__ unsigned long long trunc_abs_u64(float ax) {
@@ -1134,7 +1178,89 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
   at4(nxt.Jp, os) = jp;
 }
 
+#if AEMH_DET
+// ---- Reference-order FP32 P2G (the serial substep, mpm_solver.rs:831-903) --
+// The reference adds each node's contributions in particle-list order
+// starting from 0.0, so its float sums have exactly one order.  k_ser_emit
+// computes every contribution as the reference does (same expressions,
+// same skips) and keys it by (node, list position); after a radix sort of
+// the keys, k_ser_sum adds each node's entries in that order.
+template <int STORE>
+__global__ __launch_bounds__(256) void k_ser_emit(PBuf P, const SceneDev* __restrict__ sc, int ns,
+                                                  const MatDev* __restrict__ mats, int nmat,
+                                                  uint32_t n, SerialDev sd, uint32_t nodes) {
+  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+  if (j >= n) return;
+  const SceneDev S = sc[scene_of(sc, ns, j)];
+  const Stencil st = make_stencil(S, P.x[0][j], P.x[1][j], P.x[2][j]);
+  P2GPart q;
+  p2g_inputs<STORE>(P, j, S, st, mats, nmat, !S.vel_only, q, TableLn{sd.lntab});
+  const unsigned long long id = P.id[j];
+  const size_t e0 = (size_t)j * 27u;
+  const unsigned long long unused = (unsigned long long)nodes << sd.id_bits;
+  for (int k = 0; k < 27; k++) {
+    sd.key[0][e0 + k] = unused;
+    sd.val[0][e0 + k] = (uint32_t)(e0 + k);
+  }
+  const uint32_t boff = S.block_off;
+  const int nb0 = S.nb[0], nb1 = S.nb[1];
+  auto put = [&](int f, int di, int dj, int dk, int gx, int gy, int gz, float v) {
+    const size_t e = e0 + (size_t)((dk * 3 + dj) * 3 + di);
+    if (f == 0) {
+      const uint32_t b = boff + (uint32_t)(((gz >> 2) * nb1 + (gy >> 2)) * nb0 + (gx >> 2));
+      const uint32_t node = (b << 6) | (uint32_t)(((gz & 3) << 4) | ((gy & 3) << 2) | (gx & 3));
+      sd.key[0][e] = ((unsigned long long)node << sd.id_bits) | id;
+      if (S.vel_only)  // the velocity-only tier leaves the force at zero
+        for (int g = 4; g < 8; g++) sd.contrib[e * 8 + g] = 0.0f;
+    }
+    sd.contrib[e * 8 + f] = v;
+  };
+  if (S.vel_only)
+    p2g_nodes<false>(S.n[0], S.n[1], S.n[2], S.dx, S.o[0], S.o[1], S.o[2], st, q, put);
+  else
+    p2g_nodes<true>(S.n[0], S.n[1], S.n[2], S.dx, S.o[0], S.o[1], S.o[2], st, q, put);
+}
+
+// One thread per sorted entry; the first entry of each node adds up the
+// node's run in list order and writes its 7 sums.  Nodes no particle
+// reaches keep the zeros the scatter kernel wrote.
+__global__ __launch_bounds__(256) void k_ser_sum(SerialDev sd, uint32_t m, uint32_t nodes, GridDev G) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= m) return;
+  const unsigned long long* key = sd.key[1];
+  const unsigned long long node = key[i] >> sd.id_bits;
+  if (node >= nodes) return;
+  if (i > 0 && (key[i - 1] >> sd.id_bits) == node) return;
+  float sum[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  for (uint32_t k = i; k < m && (key[k] >> sd.id_bits) == node; k++) {
+    const float* c = sd.contrib + (size_t)sd.val[1][k] * 8;
+#pragma unroll
+    for (int f = 0; f < 7; f++) sum[f] += c[f];
+  }
+#pragma unroll
+  for (int f = 0; f < 7; f++) G.accf[(size_t)f * G.nodes + node] = sum[f];
+}
+#endif
+
 }  // namespace
+
+#if AEMH_DET
+void launch_p2g_serial(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, const MatDev* mats,
+                       int nmat, uint32_t n, SerialDev sd, GridDev g, int storage) {
+  if (!n) return;
+  const unsigned grid = (n + 255u) / 256u;
+  const uint32_t m = n * 27u;
+  if (storage == 1)
+    launch_k(k_ser_emit<1>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, sd, g.nodes);
+  else if (storage == 2)
+    launch_k(k_ser_emit<2>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, sd, g.nodes);
+  else
+    launch_k(k_ser_emit<0>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, sd, g.nodes);
+  launch_sort_pairs_u64(st, sd.tmp, sd.tmp_bytes, sd.key[0], sd.key[1], sd.val[0], sd.val[1], m,
+                        sd.end_bit);
+  launch_k(k_ser_sum, (m + 255u) / 256u, 256, 0, st, sd, m, g.nodes, g);
+}
+#endif
 
 template <int STORE>
 void launch_p2g_store(hipStream_t st, int tiles, int partials, unsigned grid, PBuf cur,

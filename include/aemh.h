@@ -56,8 +56,15 @@ enum {
 };
 
 /* P2G accumulation: FP32 atomics, or Q32.32 int64 (FixedPoint format of
- * ae_physics/src/fixed_point.rs:4-38) -- the deterministic mode. */
-enum { AEMH_ACCUM_F32 = 0, AEMH_ACCUM_Q32_32 = 1 };
+ * ae_physics/src/fixed_point.rs:4-38) -- the deterministic mode -- or FP32
+ * sums in the reference serial substep's order (mpm_solver.rs:831-903, the
+ * path the reference takes below 50,000 indices, :1077-1081): each node's
+ * contributions added in index-list order, the reference's per-node G2P
+ * order, and ln(J) from the host C library's logf (as Rust's f32::ln) over
+ * [0.1, 10] (a 223 MB table built on first use).  Bit-exact with a CPU run of
+ * the reference's serial FP32 arithmetic for the reference's material models;
+ * the snow model's hardening exp (an addition) is the portable one. */
+enum { AEMH_ACCUM_F32 = 0, AEMH_ACCUM_Q32_32 = 1, AEMH_ACCUM_F32_SERIAL = 2 };
 /* G2P x/z clamp: reference serial path [0,(n-1)dx] (mpm_solver.rs:1002-1015),
  * parallel path [o, o+(n-1)dx] (mpm_solver.rs:1287-1290), or AUTO = the
  * reference's own switch at 50,000 indices (mpm_solver.rs:1077). */
