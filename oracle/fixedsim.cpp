@@ -29,8 +29,19 @@
 //     and rebuilt F in exact integers on a quarter-turned diagonal F;
 //     brittle's config switch; fracture on F = 1.2 I (fragment count,
 //     ring offsets, masses, ids, connections, event, no second break).
+//   tests/test_oracle_pins.py: the momentum's stress term S rel / cell^2
+//     (mpm.rs:322-324) with a non-symmetric S, the column layout of the
+//     velocity gradient and F + (grad v F) dt (:384-419) against their
+//     closed forms on one particle, and update_deformation's Kirchhoff
+//     stress (:429-454) on diagonal and non-symmetric F.
 // The grid scatter bound it reports is the device's quantity, compared
 // exactly in the GPU tests.
+//
+// Planted mistakes, compiled in only by tests/test_oracle_pins.py (one
+// -DORACLE_MUT_* flag per throw-away build) to show the pins fail on them:
+//   ORACLE_MUT_FX_STRESS_T     momentum with S^T rel
+//   ORACLE_MUT_FX_VGRAD_ROWS   velocity gradient accumulated row-wise
+//   ORACLE_MUT_FX_KIRCHHOFF_F  Kirchhoff stress times F instead of F^T
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
@@ -429,7 +440,11 @@ struct Sim {
             if (w.raw < FP_EPS.raw) continue;
             V3 gp = v3(mul(from_i32(gx), cell), mul(from_i32(gy), cell), mul(from_i32(gz), cell));
             V3 rel = vsub(gp, q.pos);
+#ifdef ORACLE_MUT_FX_STRESS_T
+            V3 str = mvmul(mtrans(q.stress), rel);
+#else
             V3 str = mvmul(q.stress, rel);
+#endif
             V3 mom = vmul(vadd(q.vel, vmul(vmul(str, inv), inv)), q.mass);
             const Fx dm = mul(q.mass, w);
             const V3 dv = vmul(mom, w);
@@ -475,9 +490,16 @@ struct Sim {
               V3 col = vg.c[a];
               for (int b = 0; b < 3; b++) {
                 Fx contrib = mul(mul(mul(gva, w), elc(gd, b)), inv);
+#ifdef ORACLE_MUT_FX_VGRAD_ROWS
+                el(vg.c[b], a) = add(elc(vg.c[b], a), contrib);
+                (void)col;
+#else
                 el(col, b) = add(elc(col, b), contrib);
+#endif
               }
+#ifndef ORACLE_MUT_FX_VGRAD_ROWS
               vg.c[a] = col;
+#endif
             }
           }
       q.vel = nv;
@@ -497,7 +519,11 @@ struct Sim {
     Fx jc = fclamp(j, from_f32(0.01f), from_f32(100.0f));
     M3 finvt = mtrans(minv(f));
     M3 pk = madd(mmuls(msub(f, finvt), mu), mmuls(finvt, mul(lambda, fln(jc))));
+#ifdef ORACLE_MUT_FX_KIRCHHOFF_F
+    return mdivs(mmul(pk, f), jc);
+#else
     return mdivs(mmul(pk, mtrans(f)), jc);
+#endif
   }
   void update_deformation() {
     Fx mu, lambda;

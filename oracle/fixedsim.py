@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfixedsim.so")
+# AEMH_ORACLE_FIXEDSIM names a throw-away build with a planted mistake (only
+# tests/test_oracle_pins.py sets it, to show the pins fail on it).
+LIB_PATH = os.environ.get("AEMH_ORACLE_FIXEDSIM") or os.path.join(_HERE, "libfixedsim.so")
 
 ELASTIC, ELASTOPLASTIC, GRANULAR, BRITTLE, FLUID, SNOW = range(6)
 ONE = 1 << 32

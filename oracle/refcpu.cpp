@@ -17,7 +17,10 @@
 //                                       closed forms)
 //   P2G / grid update / G2P .......... pinned (closed forms: free fall,
 //                                       partition deficit, linear-field APIC,
-//                                       mass/momentum conservation)
+//                                       mass/momentum conservation; the 1-D
+//                                       force derivative and its update term,
+//                                       grad v and F' = (I + dt grad v) F on
+//                                       one particle, tests/test_oracle_pins.py)
 //   snow clamp + SVD (no reference) .. pinned (reconstruction/orthogonality,
 //                                       known singular values)
 //   det_logf / det_expf (no reference) pinned against libm double log/exp
@@ -25,6 +28,17 @@
 //                                       Rust toolchain exists to run the
 //                                       reference itself.
 #include "refcpu.h"
+
+// Planted mistakes, compiled in only by tests/test_oracle_pins.py (one
+// -DORACLE_MUT_* flag per build of a throw-away copy) to show that the
+// closed-form pins fail on them; the oracle proper is built without any.
+//   ORACLE_MUT_TEXTBOOK_P2G   P2G force with the tensor-product gradient
+//                             N'(x)N(y)N(z) instead of the 1-D N'(x) (Q3)
+//   ORACLE_MUT_NO_FORCE_TERM  grid update without f/m (mpm_solver.rs:127-129)
+//   ORACLE_MUT_TEXTBOOK_G2P   G2P grad v = -sum g (x) grad w instead of
+//                             -sum w g (x) N' (Q4)
+//   ORACLE_MUT_TRANSPOSE_GV   grad v stored transposed
+//   ORACLE_MUT_F_RIGHT        F' = F (I + dt grad v) instead of (I + dt grad v) F
 
 #include <math.h>
 #include <string.h>
@@ -681,6 +695,11 @@ void p2g_particle(Grid<R>& G, R* gm, R* gv, R* gf, int64_t* gq, int q32,
         R dwx = b_spline_deriv((R)di - s.fx[0]) * G.inv_dx;
         R dwy = b_spline_deriv((R)dj - s.fx[1]) * G.inv_dx;
         R dwz = b_spline_deriv((R)dk - s.fx[2]) * G.inv_dx;
+#ifdef ORACLE_MUT_TEXTBOOK_P2G
+        dwx *= wy * wz;
+        dwy *= wx * wz;
+        dwz *= wx * wy;
+#endif
         R cm = w * mass;
         R cp0 = w * mass * vx, cp1 = w * mass * vy, cp2 = w * mass * vz;
         R cf0 = sv[0] * dwx + sv[1] * dwy + sv[2] * dwz;
@@ -723,9 +742,15 @@ void grid_update_range(Grid<R>& G, const RefConfig* cfg, R dt, size_t n0, size_t
     v[0] /= m;
     v[1] /= m;
     v[2] /= m;
+#ifdef ORACLE_MUT_NO_FORCE_TERM
+    v[0] += g0 * dt;
+    v[1] += g1 * dt;
+    v[2] += g2 * dt;
+#else
     v[0] += (g0 + f[0] / m) * dt;
     v[1] += (g1 + f[1] / m) * dt;
     v[2] += (g2 + f[2] / m) * dt;
+#endif
     int64_t j = ((int64_t)gid % (nx * ny)) / nx;
     if (j == 0) {
       v[1] = rmax(v[1], R(0.0));
@@ -783,6 +808,19 @@ void g2p_particle(const Grid<R>& G, const RefConfig* cfg, int window_clamp, R dt
         R dwx = b_spline_deriv((R)di - s.fx[0]) * G.inv_dx;
         R dwy = b_spline_deriv((R)dj - s.fx[1]) * G.inv_dx;
         R dwz = b_spline_deriv((R)dk - s.fx[2]) * G.inv_dx;
+#if defined(ORACLE_MUT_TEXTBOOK_G2P)
+        for (int r = 0; r < 3; r++) {
+          gv[3 * r + 0] -= g[r] * (dwx * wy * wz);
+          gv[3 * r + 1] -= g[r] * (dwy * wx * wz);
+          gv[3 * r + 2] -= g[r] * (dwz * wx * wy);
+        }
+#elif defined(ORACLE_MUT_TRANSPOSE_GV)
+        for (int r = 0; r < 3; r++) {
+          gv[r + 0] -= w * g[r] * dwx;
+          gv[r + 3] -= w * g[r] * dwy;
+          gv[r + 6] -= w * g[r] * dwz;
+        }
+#else
         gv[0] -= w * g[0] * dwx;
         gv[1] -= w * g[0] * dwy;
         gv[2] -= w * g[0] * dwz;
@@ -792,6 +830,7 @@ void g2p_particle(const Grid<R>& G, const RefConfig* cfg, int window_clamp, R dt
         gv[6] -= w * g[2] * dwx;
         gv[7] -= w * g[2] * dwy;
         gv[8] -= w * g[2] * dwz;
+#endif
       }
   // Q7: non-finite velocity zeroes v and C (grad v is kept), then +-50 clamp.
   for (int a = 0; a < 3; a++) {
@@ -859,7 +898,12 @@ void g2p_particle(const Grid<R>& G, const RefConfig* cfg, int window_clamp, R dt
       for (int k = 0; k < 3; k++) {
         R kron = (r == k) ? R(1.0) : R(0.0);
         R term = kron + gv[3 * r + k] * dt;
+#ifdef ORACLE_MUT_F_RIGHT
+        sum += F[3 * r + k] * ((k == cc ? R(1.0) : R(0.0)) + gv[3 * k + cc] * dt);
+        (void)term;
+#else
         sum += term * F[3 * k + cc];
+#endif
       }
       nf[3 * r + cc] = sum;
     }

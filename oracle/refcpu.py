@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "librefcpu.so")
+# AEMH_ORACLE_REFCPU names a throw-away build with a planted mistake (only
+# tests/test_oracle_pins.py sets it, to show the pins fail on it).
+LIB_PATH = os.environ.get("AEMH_ORACLE_REFCPU") or os.path.join(_HERE, "librefcpu.so")
 
 NEO_HOOKEAN, DRUCKER_PRAGER, VON_MISES, NEWTONIAN_FLUID, COROTATED, FIXED_COROTATED, RIGID, SNOW = range(8)
 ACCUM_F32, ACCUM_Q32 = 0, 1
