@@ -76,7 +76,8 @@ class Config(C.Structure):
         ("accum", C.c_uint32),
         ("clamp_mode", C.c_uint32),
         ("storage", C.c_uint32),
-        ("reserved", C.c_uint32 * 4),
+        ("resort_interval", C.c_uint32),
+        ("reserved", C.c_uint32 * 3),
     ]
 
 
@@ -148,7 +149,7 @@ EXPORTS = [
     "aemh_step_velocity_only", "aemh_set_scenes", "aemh_batch_step", "aemh_autotune",
     "aemh_get_launch_shape", "aemh_set_launch_shape",
     "aemh_download_grid", "aemh_download_grid_q32", "aemh_conservation_f64",
-    "aemh_conservation_f64_noalloc",
+    "aemh_conservation_f64_noalloc", "aemh_svd_fp64_count",
     "aemh_last_substep_info", "aemh_profile_enable", "aemh_profile_get", "aemh_profile_reset",
     "aemh_scene_uniform_box",
     "aemh_mpss_reserve", "aemh_mpss_spawn", "aemh_mpss_kill", "aemh_mpss_compact",
@@ -208,6 +209,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.aemh_download_grid_q32.argtypes = [P, u32, C.POINTER(C.c_int64)]
     lib.aemh_conservation_f64.argtypes = [P, C.POINTER(C.c_double)]
     lib.aemh_conservation_f64_noalloc.argtypes = [P, C.POINTER(C.c_double)]
+    lib.aemh_svd_fp64_count.argtypes = [P, C.POINTER(C.c_uint64)]
     lib.aemh_last_substep_info.argtypes = [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     lib.aemh_profile_enable.argtypes = [P, C.c_int]
     lib.aemh_profile_get.argtypes = [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
@@ -471,6 +473,12 @@ class Engine:
         out = (C.c_double * 5)()
         self._check(self._lib.aemh_conservation_f64_noalloc(self._h, out))
         return np.array(out[:], np.float64)
+
+    def svd_fp64_count(self) -> int:
+        """Snow SVDs that took the FP64 path since the particles were uploaded."""
+        v = C.c_uint64()
+        self._check(self._lib.aemh_svd_fp64_count(self._h, C.byref(v)))
+        return int(v.value)
 
     def last_substep_info(self):
         """(active 4^3 node blocks, particles) of the last substep."""

@@ -812,8 +812,10 @@ aemh_status run_substeps(aemh_ctx* c, uint32_t k, bool allow_graph = true) {
   for (uint32_t i = 0; i < k; i++) {
     // Re-sort on the first substep of a new working set, when the block
     // order has decayed, and at least every kResortMax substeps.
-    const bool resort = c->need_sort || c->since_sort + 1 >= kResortMax ||
-                        2 * store_bytes(c->n_work, c->work[0].storage) <= kInfinityCacheBytes;
+    const uint32_t every = c->cfg.resort_interval;
+    const bool resort = every ? (c->need_sort || c->since_sort + 1 >= every)
+                              : (c->need_sort || c->since_sort + 1 >= kResortMax ||
+                                 2 * store_bytes(c->n_work, c->work[0].storage) <= kInfinityCacheBytes);
     counted_sorted = c->fresh_sort;
     if (graph) {
       hipGraphExec_t& ge = c->gexec[c->cur * 2 + (resort ? 1 : 0)];
@@ -1635,6 +1637,18 @@ aemh_status aemh_conservation_f64_noalloc(aemh_ctx* c, double* out5) {
     e = hipMemcpyAsync(out5, c->cons_scratch, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return fail(c, AEMH_ERR_HIP, "reduction: %s", hipGetErrorName(e));
+  return AEMH_OK;
+}
+
+aemh_status aemh_svd_fp64_count(aemh_ctx* c, uint64_t* count) {
+  if (!c || !count) return fail(c, AEMH_ERR_ARG, "null argument");
+  hipSetDevice(c->device);
+  uint32_t v = 0;
+  if (c->sort.counters) {
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    HIPTRY(c, hipMemcpy(&v, c->sort.counters + kCntSvd64, 4, hipMemcpyDeviceToHost));
+  }
+  *count = v;
   return AEMH_OK;
 }
 

@@ -100,6 +100,9 @@ constexpr int kDetFields = 8;
 // overflow flag (a node's sum |contribution| reached 2^63 raw).
 constexpr int kCntActive = 0;
 constexpr int kCntOverflow = 4;
+// counters[5]: snow SVDs that took the FP64 path since the particles were
+// uploaded (cumulative; the scan clears only counters[0..3])
+constexpr int kCntSvd64 = 5;
 
 // Per-substep bookkeeping buffers.
 struct SortDev {

@@ -354,12 +354,15 @@ __device__ __noinline__ void svd3_jacobi_f64(const double* F, double* U, double*
   svd3_jacobi_body<double>(F, U, sig, V);
 }
 
-// FP32 SVD with the FP64 fallback for cond(F) > 1e4.
-__device__ __forceinline__ void svd3(const float* F, float* U, float* sig, float* V) {
+// FP32 SVD with the FP64 fallback for cond(F) > 1e4 (counted in *fp64_count
+// when given).
+__device__ __forceinline__ void svd3(const float* F, float* U, float* sig, float* V,
+                                     uint32_t* fp64_count = nullptr) {
   svd3_jacobi_body<float>(F, U, sig, V);
   float smax = fmaxf(fabsf(sig[0]), fmaxf(fabsf(sig[1]), fabsf(sig[2])));
   float smin = fminf(fabsf(sig[0]), fminf(fabsf(sig[1]), fabsf(sig[2])));
   if (!(smin * 1e4f >= smax)) {
+    if (fp64_count) atomicAdd(fp64_count, 1u);
     double Fd[9], Ud[9], sd[3], Vd[9];
     for (int i = 0; i < 9; i++) Fd[i] = F[i];
     svd3_jacobi_f64(Fd, Ud, sd, Vd);
@@ -377,7 +380,8 @@ __device__ __forceinline__ void svd3(const float* F, float* U, float* sig, float
 // clamped, which the Gershgorin discs of F^T F (eigenvalues sigma^2) decide
 // without an SVD when they lie inside [lo^2, hi^2].  No reference counterpart
 // in the FP32 solver.
-__device__ __forceinline__ void snow_clamp(const MatDev& M, float* F, float* Jp) {
+__device__ __forceinline__ void snow_clamp(const MatDev& M, float* F, float* Jp,
+                                           uint32_t* fp64_count = nullptr) {
   const float lo = 1.0f - M.theta_c, hi = 1.0f + M.theta_s;
   {
     const float c00 = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
@@ -393,7 +397,7 @@ __device__ __forceinline__ void snow_clamp(const MatDev& M, float* F, float* Jp)
       return;
   }
   float U[9], s[3], V[9];
-  svd3(F, U, s, V);
+  svd3(F, U, s, V, fp64_count);
   float sh[3];
   bool clamped = false;
 #pragma unroll

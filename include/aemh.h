@@ -86,7 +86,13 @@ typedef struct {
   uint32_t accum;          /* AEMH_ACCUM_*                               */
   uint32_t clamp_mode;     /* AEMH_CLAMP_*                               */
   uint32_t storage;        /* AEMH_STORE_*                               */
-  uint32_t reserved[4];
+  uint32_t resort_interval;/* 0: automatic; k >= 1: G2P re-sorts the
+                              working set by block every k-th substep and
+                              updates it in place in between (results do
+                              not depend on it in the Q32.32 and
+                              reference-order modes).  No reference
+                              counterpart (a tuning knob)               */
+  uint32_t reserved[3];
 } aemh_config;
 
 /* One independent scene of a batched step: a contiguous range of the active
@@ -296,6 +302,10 @@ aemh_status aemh_conservation_f64(aemh_ctx* ctx, double* out5);
  * AEMH_ERR_STATE, since gathering one allocates).  All calls on one device
  * share the buffer: calls that overlap on one device race. */
 aemh_status aemh_conservation_f64_noalloc(aemh_ctx* ctx, double* out5);
+
+/* Snow-model SVDs that took the FP64 path (cond(F) > 1e4) since the
+ * particles were uploaded.  No reference counterpart (a diagnostic). */
+aemh_status aemh_svd_fp64_count(aemh_ctx* ctx, uint64_t* count);
 
 /* Work of the last substep: active 4^3 node blocks (grid nodes touched =
  * 64 x blocks) and particles advanced. */

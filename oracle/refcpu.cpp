@@ -206,7 +206,7 @@ float det_expf(float x) {
 template <class R>
 inline R r_ln(R x, int det);
 template <>
-inline float r_ln<float>(float x, int det) { return det ? det_logf(x) : logf(x); }
+inline float r_ln<float>(float x, int det) { return det == 1 ? det_logf(x) : logf(x); }
 template <>
 inline double r_ln<double>(double x, int) { return log(x); }
 template <class R>
@@ -979,7 +979,7 @@ int step_impl(const RefConfig* cfg, const RefMaterials* mats, uint32_t n, R* pos
   for (uint32_t t = 0; t < n_idx; t++)
     if (idx[t] >= n) return -2;
   const int q32 = cfg->accum == REF_ACCUM_Q32;
-  const int det = cfg->det_math ? 1 : 0;
+  const int det = cfg->det_math <= 2 ? (int)cfg->det_math : 1;
   int window = cfg->clamp_mode == REF_CLAMP_WINDOW ||
                (cfg->clamp_mode == REF_CLAMP_AUTO && n_idx >= 50000);
   // sub_dt = total_dt / substeps as f32 (mpm_solver.rs:1075)

@@ -48,7 +48,11 @@ typedef struct {
   float bc_restitution;
   uint32_t accum;         // REF_ACCUM_*
   uint32_t clamp_mode;    // REF_CLAMP_*
-  uint32_t det_math;      // 1: portable ln/exp (deterministic mode)
+  uint32_t det_math;      // 0: libm logf/expf (as the reference calls them);
+                          // 1: portable ln and exp (deterministic mode);
+                          // 2: libm logf, portable exp (the reference-order
+                          //    FP32 mode: exp only appears in the snow model,
+                          //    which has no reference counterpart)
   uint32_t threads;       // <=1: serial; >1: fold/reduce P2G + parallel G2P
 } RefConfig;
 

@@ -8,7 +8,7 @@ Rust and cannot be run here; these vectors freeze the oracle's answers so a
 later change to it, or a GPU run on a box without the build tree, is checked
 against the same numbers.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [name ...]   (default: every fixture)
 """
 import os
 import sys
@@ -34,8 +34,13 @@ def uniform(seed, n, lo, hi):
     return rng.uniform(lo, hi, (n, 3)).astype(np.float32)
 
 
+ONLY = set(sys.argv[1:])
+
+
 def case(name, pos, mass, mat, params, models, ext, grid, dt, steps, accum, vel=None, F0=None,
-         velocity_only=False, bc_margin=3, gravity=(0.0, -9.8, 0.0)):
+         velocity_only=False, bc_margin=3, gravity=(0.0, -9.8, 0.0), J0=None):
+    if ONLY and name not in ONLY:
+        return
     nx, ny, nz, dx, origin = grid
     cfg = R.make_config(nx, ny, nz, dx, origin, 1, gravity, 0.995, bc_margin, 0.0, accum,
                         R.CLAMP_AUTO, threads=1)
@@ -43,7 +48,8 @@ def case(name, pos, mass, mat, params, models, ext, grid, dt, steps, accum, vel=
                     np.array(ext, np.float32))
     n = pos.shape[0]
     F0 = np.tile(np.eye(3, dtype=np.float32).reshape(9), (n, 1)) if F0 is None else F0
-    J0 = np.linalg.det(F0.reshape(-1, 3, 3).astype(np.float64)).astype(np.float32)
+    if J0 is None:
+        J0 = np.linalg.det(F0.reshape(-1, 3, 3).astype(np.float64)).astype(np.float32)
     st = R.State(pos, vel=vel, F9=F0, J=J0, mass=mass, mat=mat)
     for _ in range(steps):
         if velocity_only:
@@ -84,6 +90,29 @@ def main():
          R.ACCUM_Q32, vel=vel, F0=F0)
     case("velonly512_q32_10", pos, mass, mat, [NH], [R.NEO_HOOKEAN], [SNOW_EXT], grid, 1e-4, 10,
          R.ACCUM_Q32, vel=vel, velocity_only=True)
+    # snow with ill-conditioned F: a third of the particles at cond(F) 1e4,
+    # 1e6 and 1e8 (F = R1 diag(1.5, 1, 1.5 / cond) R2^T, stored J = 1), so
+    # the snow clamp's SVD takes its FP64 path
+    F0 = ill_conditioned_f(n, 4)
+    case("snow_ill512_q32_5", pos_s, mass_s, mat, [SNOW], [7], [SNOW_EXT], grid, 5e-5, 5,
+         R.ACCUM_Q32, F0=F0, J0=np.ones(n, np.float32))
+
+
+def rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    return q if np.linalg.det(q) > 0 else -q
+
+
+def ill_conditioned_f(n, seed):
+    """Identity F, except every third particle: F = R1 diag(1.5, 1, 1.5 / c) R2^T
+    with c cycling through 1e4, 1e6, 1e8."""
+    rng = np.random.default_rng(seed)
+    F = np.tile(np.eye(3), (n, 1, 1))
+    for i in range(0, n, 3):
+        c = (1e4, 1e6, 1e8)[(i // 3) % 3]
+        F[i] = rotation(rng) @ np.diag([1.5, 1.0, 1.5 / c]) @ rotation(rng).T
+    return F.reshape(n, 9).astype(np.float32)
 
 
 if __name__ == "__main__":

@@ -2,8 +2,9 @@
 
 CPU: the oracle reproduces every fixture bit-exactly (single-threaded, the
 reference's serial summation order).  GPU: the HIP path reproduces the
-Q32.32 fixtures bit-exactly and the FP32 fixture within FP32 summation-order
-noise, reading only the committed vectors.
+Q32.32 fixtures bit-exactly, and the FP32 fixture bit-exactly in the
+reference-order FP32 mode and within FP32 summation-order noise in the
+default FP32 mode, reading only the committed vectors.
 """
 import glob
 import os
@@ -29,7 +30,8 @@ def load(path):
 
 def test_fixtures_present():
     names = {os.path.basename(p)[:-4] for p in FIXTURES}
-    assert {"cube512_nh_q32_10", "cube512_nh_f32_10", "snow512_q32_20", "velonly512_q32_10"} <= names
+    assert {"cube512_nh_q32_10", "cube512_nh_f32_10", "snow512_q32_20", "velonly512_q32_10",
+            "snow_ill512_q32_5"} <= names
 
 
 @pytest.mark.parametrize("path", FIXTURES, ids=lambda p: os.path.basename(p)[:-4])
@@ -50,7 +52,7 @@ def test_oracle_reproduces_fixture(path):
         assert np.array_equal(got, d[f].reshape(got.shape)), f
 
 
-def _gpu_run(d):
+def _gpu_run(d, accum=None):
     params, models, ext = d["params"], d["models"], d["ext"]
     mats = [aemh.Material(*[float(x) for x in params[i]], model=int(models[i]),
                           theta_c=float(ext[i][0]), theta_s=float(ext[i][1]), xi=float(ext[i][2]))
@@ -61,6 +63,8 @@ def _gpu_run(d):
     cfg.damping = 0.995
     cfg.bc_margin = int(d["bc_margin"])
     cfg.accum = aemh.ACCUM_Q32_32 if int(d["accum"]) == R.ACCUM_Q32 else aemh.ACCUM_F32
+    if accum is not None:
+        cfg.accum = accum
     eng = aemh.Engine()
     eng.set_config(cfg)
     nx, ny, nz = (int(v) for v in d["grid_n"])
@@ -88,7 +92,13 @@ def test_gpu_reproduces_fixture(path):
             g = got[f].reshape(want.shape)
             assert np.array_equal(g, want), f"{f}: {int((g != want).sum())} values differ"
     else:
-        # FP32: only the P2G summation order differs; 10 steps of the cube
+        # the reference's serial order: bit for bit
+        ser = _gpu_run(d, aemh.ACCUM_F32_SERIAL)
+        for f in OUT:
+            want = d[f]
+            g = ser[f].reshape(want.shape)
+            assert np.array_equal(g, want), f"serial {f}: {int((g != want).sum())} values differ"
+        # default FP32: only the P2G summation order differs; 10 steps of the cube
         for f in ("pos", "vel", "F"):
             want = d[f].astype(np.float64)
             err = np.linalg.norm(got[f].reshape(want.shape) - want) / np.linalg.norm(want)
