@@ -75,3 +75,37 @@ def test_create_without_gpu_fails_cleanly():
         assert e.status in (aemh.ERR_NO_DEVICE, aemh.ERR_HIP)
     else:
         raise AssertionError("Engine() must fail without a gfx950 device")
+
+
+def _param_counts(src, pattern):
+    """name -> number of parameters of every declaration matching pattern."""
+    out = {}
+    for m in re.finditer(pattern, src):
+        depth, i = 1, m.end()
+        while depth:
+            depth += {"(": 1, ")": -1}.get(src[i], 0)
+            i += 1
+        args = src[m.end():i - 1].strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_rust_binding_declares_every_entry_point():
+    """ae_mpm_hip (the Rust crate a maintainer adds to the workspace; there is
+    no cargo here) declares every function of include/aemh.h with the same
+    number of parameters, and its aemh_config / aemh_scene mirrors have the
+    header's fields in order."""
+    hdr = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "aemh.h")).read(), flags=re.S)
+    rs = open(os.path.join(ROOT, "ae-engine_amd", "ae_mpm_hip", "src", "lib.rs")).read()
+    want = _param_counts(hdr, r"\b(aemh_[a-z0-9_]+)\s*\(")
+    got = _param_counts(rs, r"pub fn (aemh_[a-z0-9_]+)\s*\(")
+    assert got == want
+    for c_name, rs_name in (("aemh_config", "AemhConfig"), ("aemh_scene", "AemhScene")):
+        c_body = re.search(r"typedef struct \{([^}]*)\} " + c_name + ";", hdr).group(1)
+        c_fields = []
+        for decl in c_body.split(";")[:-1]:
+            names = decl.split(None, 1)[1]  # drop the type
+            c_fields += [re.match(r"\s*([a-z_0-9]+)", d).group(1) for d in names.split(",")]
+        rs_body = re.search(r"pub struct " + rs_name + r" \{(.*?)\}", rs, re.S).group(1)
+        rs_fields = re.findall(r"pub ([a-z_0-9]+):", rs_body)
+        assert c_fields == rs_fields, (c_name, c_fields, rs_fields)
