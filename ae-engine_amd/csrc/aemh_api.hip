@@ -176,6 +176,12 @@ struct aemh_ctx {
 
 namespace {
 
+// The grid's accumulator plane layout (kAcc*) of the configured mode.
+static_assert(kAccF32Fold == AEMH_ACCUM_F32 && kAccQ32 == AEMH_ACCUM_Q32_32 &&
+                  kAccF32Mpf == AEMH_ACCUM_F32_SERIAL,
+              "plane layouts are numbered as the accumulation modes");
+int acc_layout(const aemh_ctx* c) { return (int)c->cfg.accum; }
+
 // This device's address of `scratch`, looked up once per device with the
 // device current (each device holds its own copy at its own address).
 hipError_t scratch_for_device(int device, double** out) {
@@ -728,7 +734,7 @@ void enqueue_substep(aemh_ctx* c, int cur, bool resort) {
   hipStream_t st = c->stream;
   timed_k(c, "count", [&] { launch_count(st, a, c->scd_d, ns, n, c->sort, c->nb_total); });
   timed_k(c, "scan", [&] { launch_scan(st, c->scd_d, ns, c->sort, c->nb_total); });
-  timed_k(c, "scatter", [&] { launch_scatter(st, n, c->nb_total, c->sort, c->grid, q32); });
+  timed_k(c, "scatter", [&] { launch_scatter(st, n, c->nb_total, c->sort, c->grid, acc_layout(c)); });
   if (c->cfg.accum == AEMH_ACCUM_F32_SERIAL) {
     // FP32 sums in list order, the FP32 grid update (atomic-flush layout:
     // no P2G partials), the reference-order G2P of the deterministic build
@@ -737,7 +743,7 @@ void enqueue_substep(aemh_ctx* c, int cur, bool resort) {
     timed_k(c, "p2g", [&] {
       det::launch_p2g_serial(st, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->ser, c->grid, storage);
     });
-    timed_k(c, "grid_update", [&] { fast::launch_grid_update(st, ls, c->scd_d, ns, c->sort, c->grid, n, c->nb_total); });
+    timed_k(c, "grid_update", [&] { fast::launch_grid_update_mpf(st, ls, c->scd_d, ns, c->sort, c->grid, n, c->nb_total); });
     timed_k(c, "g2p", [&] {
       det::launch_g2p(st, ls, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, inplace);
     });
@@ -1449,7 +1455,7 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   LaunchShape& ls = c->shape;
   launch_count(st, a, c->scd_d, ns, n, c->sort, c->nb_total);
   launch_scan(st, c->scd_d, ns, c->sort, c->nb_total);
-  launch_scatter(st, n, c->nb_total, c->sort, c->grid, q32);
+  launch_scatter(st, n, c->nb_total, c->sort, c->grid, acc_layout(c));
   CHECK_LAUNCH(c);
   constexpr int kReps = 3;
   float best = 1e30f, ms = 0.0f;
@@ -1463,7 +1469,7 @@ aemh_status aemh_autotune(aemh_ctx* c) {
       ls.p2g_tiles = tiles;
       ls.p2g_partials = part;
       s = time_launches(c, kReps, &ms, [&] {
-        launch_clear(st, c->sort, c->grid, q32, ls.upd_wgs);
+        launch_clear(st, c->sort, c->grid, acc_layout(c), ls.upd_wgs);
         if (q32) {
           det::launch_p2g(st, ls, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
           det::launch_grid_update(st, ls, c->scd_d, ns, c->sort, c->grid, n, c->nb_total);
@@ -1479,7 +1485,7 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   ls.p2g_partials = pick_part;
   c->tune_ms[0] = best;
   // Partials in the chosen tile layout for the grid-update candidates.
-  launch_clear(st, c->sort, c->grid, q32, ls.upd_wgs);
+  launch_clear(st, c->sort, c->grid, acc_layout(c), ls.upd_wgs);
   if (q32)
     det::launch_p2g(st, ls, a, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage);
   else
@@ -1563,9 +1569,8 @@ aemh_status aemh_download_grid(aemh_ctx* c, uint32_t scene, float* mass, float* 
   float* tmp = nullptr;
   HIPTRY(c, hipMalloc(&tmp, nodes * 10 * 4));
   HIPTRY(c, hipMemsetAsync(tmp, 0, nodes * 10 * 4, c->stream));
-  launch_grid_readback(c->stream, c->scd_d, (int)scene, S, c->sort, c->grid,
-                       c->cfg.accum == AEMH_ACCUM_Q32_32, tmp, tmp + nodes, tmp + 4 * nodes,
-                       tmp + 7 * nodes, nullptr, c->shape.upd_wgs);
+  launch_grid_readback(c->stream, c->scd_d, (int)scene, S, c->sort, c->grid, acc_layout(c), tmp,
+                       tmp + nodes, tmp + 4 * nodes, tmp + 7 * nodes, nullptr, c->shape.upd_wgs);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess && mass) e = hipMemcpy(mass, tmp, nodes * 4, hipMemcpyDeviceToHost);
@@ -1587,7 +1592,7 @@ aemh_status aemh_download_grid_q32(aemh_ctx* c, uint32_t scene, int64_t* q7) {
   long long* tmp = nullptr;
   HIPTRY(c, hipMalloc(&tmp, nodes * 7 * 8));
   HIPTRY(c, hipMemsetAsync(tmp, 0, nodes * 7 * 8, c->stream));
-  launch_grid_readback(c->stream, c->scd_d, (int)scene, S, c->sort, c->grid, 1, nullptr, nullptr,
+  launch_grid_readback(c->stream, c->scd_d, (int)scene, S, c->sort, c->grid, kAccQ32, nullptr, nullptr,
                        nullptr, nullptr, tmp, c->shape.upd_wgs);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

@@ -96,6 +96,10 @@ struct GridDev {
   uint32_t nodes;
 };
 constexpr int kDetFields = 8;
+// Accumulator planes of the grid, by mode (numbered as AEMH_ACCUM_*): the
+// FP32 mode's 4 f32 planes (m, p + f dt), the deterministic mode's 8 int64
+// planes, the reference-order mode's 7 f32 planes (m, p, f).
+constexpr int kAccF32Fold = 0, kAccQ32 = 1, kAccF32Mpf = 2;
 // counters[0]: active blocks of the substep; counters[4]: sticky Q32.32
 // overflow flag (a node's sum |contribution| reached 2^63 raw).
 constexpr int kCntActive = 0;

@@ -51,9 +51,9 @@ void launch_count(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, uint32_t
 // Scan of the block counts (with the halo activation flags).
 void launch_scan(hipStream_t st, const SceneDev* sc, int ns, SortDev s, uint32_t nb);
 // Block-order permutation, active-block list and the zeroing of its blocks'
-// accumulators.
-void launch_scatter(hipStream_t st, uint32_t n, uint32_t nb, SortDev s, GridDev g, int q32);
-void launch_clear(hipStream_t st, SortDev s, GridDev g, int q32, int upd_wgs);
+// accumulators (acc: the plane layout, kAcc*).
+void launch_scatter(hipStream_t st, uint32_t n, uint32_t nb, SortDev s, GridDev g, int acc);
+void launch_clear(hipStream_t st, SortDev s, GridDev g, int acc, int upd_wgs);
 // canonical (capacity) <-> working set
 void launch_gather_work(hipStream_t st, PBuf canon, PBuf work, const uint32_t* idx, uint32_t n,
                         int storage);
@@ -62,7 +62,7 @@ void launch_scatter_work(hipStream_t st, PBuf work, PBuf canon, const uint32_t* 
 void launch_fill(hipStream_t st, float* p, float v, uint32_t n);
 // Dense x-fastest readback of one scene's grid (active blocks only).
 void launch_grid_readback(hipStream_t st, const SceneDev* sc_d, int scene, SceneDev sc,
-                          SortDev s, GridDev g, int q32, float* m, float* p3, float* f3,
+                          SortDev s, GridDev g, int acc, float* m, float* p3, float* f3,
                           float* v3, long long* q7, int upd_wgs);
 // FP64 sums over the working set: mass, momentum xyz, kinetic energy.
 // out5 needs room for kConsScratch doubles: the 5 sums, then the 5 partial
@@ -127,6 +127,11 @@ void launch_gather_pos(hipStream_t st, PBuf P, const uint32_t* list, uint32_t n,
 AEMH_DECLARE_MODE(fast)
 AEMH_DECLARE_MODE(det)
 #undef AEMH_DECLARE_MODE
+namespace fast {
+// The reference-order mode's grid update on 7 FP32 planes (m, p, f).
+void launch_grid_update_mpf(hipStream_t st, const LaunchShape& ls, const SceneDev* sc, int ns,
+                            SortDev s, GridDev g, uint32_t n, uint32_t nb_total);
+}
 namespace det {
 // Reference-order FP32 P2G: the grid's FP32 sums (accf) of the active
 // blocks, each node's contributions added in list order.

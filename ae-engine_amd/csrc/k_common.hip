@@ -198,7 +198,7 @@ __global__ __launch_bounds__(256) void k_scan_tsum(SortDev s, uint32_t ntiles) {
 // accumulators, which the wave does one block at a time with coalesced
 // stores.  All of it only needs the finished scans.
 __global__ __launch_bounds__(256) void k_scatter(uint32_t n, uint32_t nb, SortDev s, GridDev g,
-                                                 int q32) {
+                                                 int acc) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   const bool on = i < nb && s.flag[i] != 0u;
   const unsigned long long m = __ballot(on);
@@ -216,10 +216,11 @@ __global__ __launch_bounds__(256) void k_scatter(uint32_t n, uint32_t nb, SortDe
     const uint32_t b0 = i - (uint32_t)lane;  // block of lane 0
     for (unsigned long long r = m; r; r &= r - 1ull) {
       const uint32_t node = ((b0 + (uint32_t)(__ffsll((long long)r) - 1)) << 6) | (uint32_t)lane;
-      if (q32) {
+      if (acc == kAccQ32) {
         for (int f = 0; f < kDetFields; f++) g.accq[(size_t)f * g.nodes + node] = 0;
       } else {
-        for (int f = 0; f < 7; f++) g.accf[(size_t)f * g.nodes + node] = 0.0f;
+        const int planes = acc == kAccF32Fold ? 4 : 7;
+        for (int f = 0; f < planes; f++) g.accf[(size_t)f * g.nodes + node] = 0.0f;
       }
     }
   }
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(256) void k_scatter(uint32_t n, uint32_t nb, SortDe
 }
 
 // Zero the accumulators of this substep's active blocks (one wave per block).
-__global__ __launch_bounds__(256) void k_clear(SortDev s, GridDev g, int q32) {
+__global__ __launch_bounds__(256) void k_clear(SortDev s, GridDev g, int acc) {
   const uint32_t lane = __lane_id();
   const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * 256u) >> 6;
@@ -238,10 +239,11 @@ __global__ __launch_bounds__(256) void k_clear(SortDev s, GridDev g, int q32) {
   for (uint32_t a = wave; a < nact; a += nwaves) {
     const uint32_t b = s.active[a];
     const uint32_t node = (b << 6) | lane;
-    if (q32) {
+    if (acc == kAccQ32) {
       for (int f = 0; f < kDetFields; f++) g.accq[(size_t)f * g.nodes + node] = 0;
     } else {
-      for (int f = 0; f < 7; f++) g.accf[(size_t)f * g.nodes + node] = 0.0f;
+      const int planes = acc == kAccF32Fold ? 4 : 7;
+      for (int f = 0; f < planes; f++) g.accf[(size_t)f * g.nodes + node] = 0.0f;
     }
   }
 }
@@ -295,7 +297,7 @@ __global__ __launch_bounds__(256) void k_fill(float* p, float v, uint32_t n) {
 }
 
 __global__ __launch_bounds__(256) void k_grid_readback(const SceneDev* __restrict__ scd,
-                                                       int scene, SortDev s, GridDev g, int q32,
+                                                       int scene, SortDev s, GridDev g, int acc,
                                                        float* m, float* p3, float* f3, float* v3,
                                                        long long* q7) {
   const SceneDev S = scd[scene];
@@ -317,20 +319,24 @@ __global__ __launch_bounds__(256) void k_grid_readback(const SceneDev* __restric
     const size_t d = (size_t)gx + (size_t)gy * S.n[0] + (size_t)gz * S.n[0] * S.n[1];
     const uint32_t node = (b << 6) | lane;
     const float4 gv = g.gv[node];
-    if (q32) {
-      float acc[7];
+    if (acc == kAccQ32) {
+      float a[7];
       for (int f = 0; f < 7; f++) {
         long long r = g.accq[(size_t)f * g.nodes + node];
         if (q7) q7[7 * d + f] = r;
-        acc[f] = fp_to_f32(r);
+        a[f] = fp_to_f32(r);
       }
-      if (m) m[d] = acc[0];
-      if (p3) for (int k = 0; k < 3; k++) p3[3 * d + k] = acc[1 + k];
-      if (f3) for (int k = 0; k < 3; k++) f3[3 * d + k] = acc[4 + k];
+      if (m) m[d] = a[0];
+      if (p3) for (int k = 0; k < 3; k++) p3[3 * d + k] = a[1 + k];
+      if (f3) for (int k = 0; k < 3; k++) f3[3 * d + k] = a[4 + k];
     } else {
+      // FP32 mode: the momentum plane holds p + f dt and there is no force plane
+      const bool fold = acc == kAccF32Fold;
       if (m) m[d] = g.accf[node];
       if (p3) for (int k = 0; k < 3; k++) p3[3 * d + k] = g.accf[(size_t)(1 + k) * g.nodes + node];
-      if (f3) for (int k = 0; k < 3; k++) f3[3 * d + k] = g.accf[(size_t)(4 + k) * g.nodes + node];
+      if (f3)
+        for (int k = 0; k < 3; k++)
+          f3[3 * d + k] = fold ? 0.0f : g.accf[(size_t)(4 + k) * g.nodes + node];
     }
     if (v3) {
       v3[3 * d + 0] = gv.x;
@@ -390,12 +396,12 @@ void launch_scan(hipStream_t st, const SceneDev* sc, int ns, SortDev s, uint32_t
   launch_k(k_scan_tiles, tiles, 256, 0, st, sc, ns, s, nb);
   launch_k(k_scan_tsum, 1, 256, 0, st, s, tiles);
 }
-void launch_scatter(hipStream_t st, uint32_t n, uint32_t nb, SortDev s, GridDev g, int q32) {
+void launch_scatter(hipStream_t st, uint32_t n, uint32_t nb, SortDev s, GridDev g, int acc) {
   const uint32_t m = n > nb ? n : nb;
-  if (m) launch_k(k_scatter, grid_for(m, 256), 256, 0, st, n, nb, s, g, q32);
+  if (m) launch_k(k_scatter, grid_for(m, 256), 256, 0, st, n, nb, s, g, acc);
 }
-void launch_clear(hipStream_t st, SortDev s, GridDev g, int q32, int upd_wgs) {
-  k_clear<<<upd_wgs, 256, 0, st>>>(s, g, q32);
+void launch_clear(hipStream_t st, SortDev s, GridDev g, int acc, int upd_wgs) {
+  k_clear<<<upd_wgs, 256, 0, st>>>(s, g, acc);
 }
 void launch_gather_work(hipStream_t st, PBuf canon, PBuf work, const uint32_t* idx, uint32_t n,
                         int storage) {
@@ -409,9 +415,9 @@ void launch_fill(hipStream_t st, float* p, float v, uint32_t n) {
   if (n) k_fill<<<grid_for(n, 256), 256, 0, st>>>(p, v, n);
 }
 void launch_grid_readback(hipStream_t st, const SceneDev* sc_d, int scene, SceneDev, SortDev s,
-                          GridDev g, int q32, float* m, float* p3, float* f3, float* v3,
+                          GridDev g, int acc, float* m, float* p3, float* f3, float* v3,
                           long long* q7, int upd_wgs) {
-  k_grid_readback<<<upd_wgs, 256, 0, st>>>(sc_d, scene, s, g, q32, m, p3, f3, v3, q7);
+  k_grid_readback<<<upd_wgs, 256, 0, st>>>(sc_d, scene, s, g, acc, m, p3, f3, v3, q7);
 }
 void launch_conservation(hipStream_t st, PBuf cur, uint32_t n, double* out5) {
   k_conservation_partial<<<kConsWgs, 256, 0, st>>>(cur, n, out5 + 5);

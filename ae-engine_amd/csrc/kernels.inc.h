@@ -25,19 +25,29 @@ namespace {
 
 // P2G stages up to TILES (2, 3 or 4; the autotuner picks) distinct base
 // blocks of one workgroup's particles in LDS; the rest scatter straight to
-// HBM.  Each tile holds kTileVals int64 accumulators (12 KB in the FP32 mode,
-// 13.5 KB in the deterministic mode), so the count trades staging coverage
-// against workgroups per CU: dense scenes (~300 particles per block) rarely
-// need a third tile, 2 particles per cell needs three.
+// HBM.  Each tile holds kTileVals int64 accumulators (6.75 KB in the FP32
+// mode, 13.5 KB in the deterministic mode), so the count trades staging
+// coverage against workgroups per CU: dense scenes (~300 particles per
+// block) rarely need a third tile, 2 particles per cell needs three.
+//
+// The FP32 mode accumulates 4 fields per node: the mass and the momentum
+// after the force impulse, p + f dt, with each particle's contribution
+// w m (v + C d) + dt sum_c vol sigma[r][c] N'_c.  The grid update then takes
+// v = (p + f dt) / m + g dt, the reference's p / m + (g + f / m) dt
+// (mpm_solver.rs:121-129) with the force divided once instead of per node;
+// the two differ by float rounding only.  The deterministic mode (and the
+// reference-order mode's 7-plane FP32 layout) keep m, p and f separate.
 #if AEMH_DET
 constexpr int kTileFields = kDetFields;  // 7 sums + the magnitude bound
+constexpr bool kFoldForce = false;
 #else
-constexpr int kTileFields = 7;
+constexpr int kTileFields = 4;           // m, (p + f dt) x 3
+constexpr bool kFoldForce = true;
 #endif
 constexpr int kTileVals = kTileFields * kTileNodes;
-// Values per tile-node record of the P2G partials (7 sums + pad, or the 8
-// deterministic-mode fields).
-constexpr int kPartStride = 8;
+// Values per tile-node record of the P2G partials (the 8 deterministic-mode
+// fields, or the 4 FP32 sums).
+constexpr int kPartStride = AEMH_DET ? 8 : 4;
 // Lane layers of the P2G re-assignment (rank within a base cell, capped).
 constexpr int kLayers = 16;
 // Base-block tiles of grid (v, m) a G2P workgroup stages in LDS.
@@ -58,8 +68,10 @@ struct P2GPart {
 
 // The 27-node scatter in the reference's dk-dj-di order; `put(f, di, dj, dk,
 // gx, gy, gz, value)` receives each contribution of field f (m, p0..2,
-// f0..2).  Full MPM multiplies (wx wy) wz, the velocity-only tier wx (wy wz).
-template <bool kFull, class Put>
+// f0..2; with kFold: m and p_r + f_r, the force term then being computed from
+// the dt-scaled stress the caller stored in sv).  Full MPM multiplies
+// (wx wy) wz, the velocity-only tier wx (wy wz).
+template <bool kFull, bool kFold = false, class Put>
 __device__ __forceinline__ void p2g_nodes(const int n0, const int n1, const int n2, const float dx,
                                           const float o0, const float o1, const float o2,
                                           const Stencil& st, const P2GPart& q, Put&& put) {
@@ -79,13 +91,22 @@ __device__ __forceinline__ void p2g_nodes(const int n0, const int n1, const int 
           const float ux = q.vx + q.c[0] * dxg + q.c[1] * dyg + q.c[2] * dzg;
           const float uy = q.vy + q.c[3] * dxg + q.c[4] * dyg + q.c[5] * dzg;
           const float uz = q.vz + q.c[6] * dxg + q.c[7] * dyg + q.c[8] * dzg;
+          const float f0 = q.sv[0] * q.dwx[di] + q.sv[1] * q.dwy[dj] + q.sv[2] * q.dwz[dk];
+          const float f1 = q.sv[3] * q.dwx[di] + q.sv[4] * q.dwy[dj] + q.sv[5] * q.dwz[dk];
+          const float f2 = q.sv[6] * q.dwx[di] + q.sv[7] * q.dwy[dj] + q.sv[8] * q.dwz[dk];
           put(0, di, dj, dk, gx, gy, gz, w * q.mass);
-          put(1, di, dj, dk, gx, gy, gz, w * q.mass * ux);
-          put(2, di, dj, dk, gx, gy, gz, w * q.mass * uy);
-          put(3, di, dj, dk, gx, gy, gz, w * q.mass * uz);
-          put(4, di, dj, dk, gx, gy, gz, q.sv[0] * q.dwx[di] + q.sv[1] * q.dwy[dj] + q.sv[2] * q.dwz[dk]);
-          put(5, di, dj, dk, gx, gy, gz, q.sv[3] * q.dwx[di] + q.sv[4] * q.dwy[dj] + q.sv[5] * q.dwz[dk]);
-          put(6, di, dj, dk, gx, gy, gz, q.sv[6] * q.dwx[di] + q.sv[7] * q.dwy[dj] + q.sv[8] * q.dwz[dk]);
+          if (kFold) {
+            put(1, di, dj, dk, gx, gy, gz, w * q.mass * ux + f0);
+            put(2, di, dj, dk, gx, gy, gz, w * q.mass * uy + f1);
+            put(3, di, dj, dk, gx, gy, gz, w * q.mass * uz + f2);
+          } else {
+            put(1, di, dj, dk, gx, gy, gz, w * q.mass * ux);
+            put(2, di, dj, dk, gx, gy, gz, w * q.mass * uy);
+            put(3, di, dj, dk, gx, gy, gz, w * q.mass * uz);
+            put(4, di, dj, dk, gx, gy, gz, f0);
+            put(5, di, dj, dk, gx, gy, gz, f1);
+            put(6, di, dj, dk, gx, gy, gz, f2);
+          }
         } else {
           put(0, di, dj, dk, gx, gy, gz, w * q.mass);
           put(1, di, dj, dk, gx, gy, gz, w * q.mass * q.vx);
@@ -219,7 +240,7 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
   __shared__ int tdim[TILES][5];  // scene grid n[0..2], nb[0..1] of the tile
   __shared__ uint32_t tboff[TILES];
   __shared__ uint32_t wcnt[4];
-  __shared__ int wexp[4][3];
+  __shared__ int wexp[4][2];
 
   const int t = threadIdx.x;
   const int lane = __lane_id(), wave = t >> 6;
@@ -341,8 +362,9 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
       float stress[9];
       compute_stress(M, F, P.J[p], P.Jp[p], stress);
       const float vol = q.mass / M.density;
+      // FP32 mode: the force term enters the momentum as its impulse f dt
 #pragma unroll
-      for (int k = 0; k < 9; k++) q.sv[k] = vol * stress[k];
+      for (int k = 0; k < 9; k++) q.sv[k] = kFoldForce ? S.dt * (vol * stress[k]) : vol * stress[k];
     } else {
 #pragma unroll
       for (int k = 0; k < 9; k++) q.c[k] = q.sv[k] = 0.0f;
@@ -351,12 +373,13 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
 
 #if !AEMH_DET
   // ---- block exponents: |contribution of group g| < 2^e_g over the WG ----
-  // mass: w m <= m; momentum: |w m u_r| <= m (|v_r| + 2 dx sum_c |C_rc|);
-  // force: |sum_c sv_rc dw_c| <= inv_dx sum_c |sv_rc|   (|N'| <= 1, |d| < 2 dx).
-  // Bounds are doubled for rounding; a non-finite bound sends the particle
-  // down the HBM path, where float atomics carry NaN/inf as the reference.
+  // mass: w m <= m; momentum after the impulse:
+  // |w m u_r + dt f_r| <= m (|v_r| + 2 dx sum_c |C_rc|) + inv_dx sum_c |dt sv_rc|
+  // (|w| <= 1, |N'| <= 1, |d| < 2 dx).  Bounds are doubled for rounding; a
+  // non-finite bound sends the particle down the HBM path, where float
+  // atomics carry NaN/inf as the reference.
   bool finite_in = true;
-  int ex[3] = {-126, -126, -126};
+  int ex[2] = {-126, -126};
   if (valid) {
     float bp = 0.0f, bf = 0.0f;
     const float vv[3] = {q.vx, q.vy, q.vz};
@@ -369,9 +392,9 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
       bp = fmaxf(bp, tp);
       bf = fmaxf(bf, tf);
     }
-    const float b[3] = {2.0f * q.mass, 2.0f * q.mass * bp, 2.0f * bf};
+    const float b[2] = {2.0f * q.mass, 2.0f * (q.mass * bp + bf)};
 #pragma unroll
-    for (int g = 0; g < 3; g++) {
+    for (int g = 0; g < 2; g++) {
       if (!(b[g] <= 3.0e38f)) finite_in = false;
       int e = 0;
       frexpf(b[g], &e);
@@ -379,7 +402,7 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
     }
   }
 #pragma unroll
-  for (int g = 0; g < 3; g++) {
+  for (int g = 0; g < 2; g++) {
     int e = finite_in ? ex[g] : -126;
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) e = max(e, __shfl_xor(e, d, 64));
@@ -389,10 +412,11 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
   __syncthreads();
 
 #if !AEMH_DET
-  int eg[3];
-  double scale[3];
+  // field f belongs to group f ? 1 : 0 (mass; momentum after the impulse)
+  int eg[2];
+  double scale[2];
 #pragma unroll
-  for (int g = 0; g < 3; g++) {
+  for (int g = 0; g < 2; g++) {
     eg[g] = max(max(wexp[0][g], wexp[1][g]), max(wexp[2][g], wexp[3][g]));
     eg[g] = max(eg[g], -70);
     scale[g] = (double)__int_as_float((127 + 50 - eg[g]) << 23);  // 2^(50 - e_g)
@@ -426,14 +450,14 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
       };
 #else
       auto put = [&](int f, int di, int dj, int dk, int, int, int, float v) {
-        const int g = f == 0 ? 0 : (f < 4 ? 1 : 2);
+        const int g = f == 0 ? 0 : 1;
         const long long a = rint_i64_scaled(v, scale[g]);  // |v scale| < 2^50
         atomicAdd((unsigned long long*)&tile[tb + f * kTileNodes + (dk * kTileSide + dj) * kTileSide + di],
                   (unsigned long long)a);
       };
 #endif
       if (full)
-        p2g_nodes<true>(n0, n1, n2, dx, o0, o1, o2, st, q, put);
+        p2g_nodes<true, kFoldForce>(n0, n1, n2, dx, o0, o1, o2, st, q, put);
       else
         p2g_nodes<false>(n0, n1, n2, dx, o0, o1, o2, st, q, put);
 #if AEMH_DET
@@ -473,7 +497,7 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
       };
 #endif
       if (full)
-        p2g_nodes<true>(n0, n1, n2, dx, o0, o1, o2, st, q, put);
+        p2g_nodes<true, kFoldForce>(n0, n1, n2, dx, o0, o1, o2, st, q, put);
       else
         p2g_nodes<false>(n0, n1, n2, dx, o0, o1, o2, st, q, put);
     }
@@ -481,8 +505,8 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
   __syncthreads();
 
   // ---- flush, partial mode: every tile goes to its partial slot with plain
-  // stores, one 8-value record per tile node (thread t < 216 owns node t);
-  // the grid update adds up the partials covering each node ----
+  // stores, one kPartStride-value record per tile node (thread t < 216 owns
+  // node t); the grid update adds up the partials covering each node ----
   if (partials && t < kTileNodes) {
     for (int ti = 0; ti < ntiles; ti++) {
       const size_t rec = (((size_t)blockIdx.x * TILES + ti) * kTileNodes + t) * kPartStride;
@@ -497,12 +521,10 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
 #pragma unroll
       for (int f = 0; f < kPartStride; f++) {
         const long long q = f < kTileFields ? tile[ti * kTileVals + f * kTileNodes + t] : 0;
-        const int g = f == 0 ? 0 : (f < 4 ? 1 : 2);
+        const int g = f == 0 ? 0 : 1;
         v[f] = q == 0 ? 0.0f : (float)((double)q * (double)__int_as_float((127 - 50 + eg[g]) << 23));
       }
-      float4* dst = reinterpret_cast<float4*>(G.partf + rec);
-      dst[0] = make_float4(v[0], v[1], v[2], v[3]);
-      dst[1] = make_float4(v[4], v[5], v[6], v[7]);
+      *reinterpret_cast<float4*>(G.partf + rec) = make_float4(v[0], v[1], v[2], v[3]);
 #endif
     }
   }
@@ -526,7 +548,7 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
 #if AEMH_DET
         atomicAdd((unsigned long long*)(G.accq + (size_t)f * nodes + node), (unsigned long long)v);
 #else
-        const int g = f == 0 ? 0 : (f < 4 ? 1 : 2);
+        const int g = f == 0 ? 0 : 1;
         const float fv = (float)((double)v * (double)__int_as_float((127 - 50 + eg[g]) << 23));
         unsafeAtomicAdd(G.accf + (size_t)f * nodes + node, fv);
 #endif
@@ -546,10 +568,15 @@ __device__ __forceinline__ uint32_t block_begin(const SortDev& s, uint32_t b, ui
 // partials of every tile covering it: tiles of the base blocks b - (i, j,
 // k), i, j, k in {0, 1}, reach this block's nodes with local coordinate < 2
 // on each shifted axis, one tile per P2G workgroup holding particles of that
-// base block.  The combined sums are written back for grid readback.
+// base block.  The combined sums are written back for grid readback.  NF is
+// the number of accumulator planes: the deterministic mode's 8, the FP32
+// mode's 4 (m and the momentum after the force impulse, the only layout
+// with partials), or the reference-order mode's 7 (m, p, f).
+template <int NF>
 __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict__ sc, int ns,
                                                      SortDev s, GridDev G, int tiles,
                                                      int partials, uint32_t n, uint32_t nb_total) {
+  constexpr bool kFolded = !AEMH_DET && NF == 4;
   const uint32_t lane = __lane_id();
   const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * 256u) >> 6;
@@ -565,13 +592,13 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
               gz = bz * 4 + (int)(lane >> 4);
     const uint32_t node = (b << 6) | lane;
 #if AEMH_DET
-    long long sum[kTileFields];
+    long long sum[NF];
 #pragma unroll
-    for (int f = 0; f < kTileFields; f++) sum[f] = G.accq[(size_t)f * G.nodes + node];
+    for (int f = 0; f < NF; f++) sum[f] = G.accq[(size_t)f * G.nodes + node];
 #else
-    float sum[kTileFields];
+    float sum[NF];
 #pragma unroll
-    for (int f = 0; f < kTileFields; f++) sum[f] = G.accf[(size_t)f * G.nodes + node];
+    for (int f = 0; f < NF; f++) sum[f] = G.accf[(size_t)f * G.nodes + node];
 #endif
     const int lx = (int)(lane & 3), ly = (int)((lane >> 2) & 3), lz = (int)(lane >> 4);
     // Sorted ranges and first-workgroup tile slots of the 8 candidate base
@@ -608,21 +635,19 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
           sum[2 * h + 1] += q.y;
         }
 #else
-        const float4* src = reinterpret_cast<const float4*>(G.partf + rec);
-        const float4 a = src[0], b2 = src[1];
-        sum[0] += a.x;
-        sum[1] += a.y;
-        sum[2] += a.z;
-        sum[3] += a.w;
-        sum[4] += b2.x;
-        sum[5] += b2.y;
-        sum[6] += b2.z;
+        if constexpr (kFolded) {
+          const float4 a = *reinterpret_cast<const float4*>(G.partf + rec);
+          sum[0] += a.x;
+          sum[1] += a.y;
+          sum[2] += a.z;
+          sum[3] += a.w;
+        }
 #endif
       }
     }
-    float acc[7];
+    float acc[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int f = 0; f < kTileFields; f++) {
+    for (int f = 0; f < NF; f++) {
 #if AEMH_DET
       G.accq[(size_t)f * G.nodes + node] = sum[f];
       if (f < 7) acc[f] = fp_to_f32(sum[f]);
@@ -642,9 +667,15 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
       v0 /= m;
       v1 /= m;
       v2 /= m;
-      v0 += (S.g[0] + acc[4] / m) * S.dt;
-      v1 += (S.g[1] + acc[5] / m) * S.dt;
-      v2 += (S.g[2] + acc[6] / m) * S.dt;
+      if constexpr (kFolded) {  // the force impulse is in the momentum already
+        v0 += S.g[0] * S.dt;
+        v1 += S.g[1] * S.dt;
+        v2 += S.g[2] * S.dt;
+      } else {
+        v0 += (S.g[0] + acc[4] / m) * S.dt;
+        v1 += (S.g[1] + acc[5] / m) * S.dt;
+        v2 += (S.g[2] + acc[6] / m) * S.dt;
+      }
       if (gy == 0) {
         v1 = fmaxf(v1, 0.0f);
         v0 *= 0.9f;
@@ -1288,9 +1319,16 @@ void launch_p2g(hipStream_t st, const LaunchShape& ls, PBuf cur, const SceneDev*
 
 void launch_grid_update(hipStream_t st, const LaunchShape& ls, const SceneDev* sc, int ns,
                         SortDev s, GridDev g, uint32_t n, uint32_t nb_total) {
-  launch_k(k_grid_update, ls.upd_wgs, 256, 0, st, sc, ns, s, g, ls.p2g_tiles, ls.p2g_partials, n,
-           nb_total);
+  launch_k(k_grid_update<kTileFields>, ls.upd_wgs, 256, 0, st, sc, ns, s, g, ls.p2g_tiles,
+           ls.p2g_partials, n, nb_total);
 }
+#if !AEMH_DET
+// The reference-order mode's update: 7 planes (m, p, f), no partials.
+void launch_grid_update_mpf(hipStream_t st, const LaunchShape& ls, const SceneDev* sc, int ns,
+                            SortDev s, GridDev g, uint32_t n, uint32_t nb_total) {
+  launch_k(k_grid_update<7>, ls.upd_wgs, 256, 0, st, sc, ns, s, g, ls.p2g_tiles, 0, n, nb_total);
+}
+#endif
 
 void launch_g2p(hipStream_t st, const LaunchShape& ls, PBuf cur, PBuf nxt, const SceneDev* sc,
                 int ns, const MatDev* mats, int nmat, uint32_t n, SortDev s, GridDev g,

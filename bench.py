@@ -8,7 +8,9 @@ P2G, grid update, G2P.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
 
-Prints one JSON line.  `value` times K steps as the library runs them: each
+Prints one JSON line.  By default W = 5 and K = 195: the timed window is
+steps 6-200 of the configuration's 200-step dam break.  `value` times K
+steps as the library runs them: each
 step replays the captured substep hipGraph and returns after a stream
 synchronize, which brackets the region.  The particles are then put back to
 their state at the start of that window and the same K steps run again with
@@ -121,34 +123,78 @@ def roofline_of(kern, name, npart, blocks, traffic=None):
             "kernel": name, "algorithmic_bytes": int(nbytes), "avg_ms": round(ms, 4)}
 
 
-def cpu_baseline(W, aemh):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def native_oracle():
+    """The oracle built for this host the way the reference builds for its own
+    (Cargo.toml:116-120: opt-level 3, fat LTO; .cargo/config.toml:
+    target-cpu=native): g++ -O3 -march=native -flto, still without FMA
+    contraction (rustc never fuses).  Built here, on the machine that runs
+    it; returns the library path, or None if the build fails."""
+    import subprocess
+    import tempfile
+    out = os.path.join(tempfile.mkdtemp(prefix="aemh_native_"), "librefcpu_native.so")
+    cmd = ["g++", "-O3", "-march=native", "-flto", "-std=c++17", "-fPIC", "-shared", "-pthread",
+           "-ffp-contract=off", "-fno-math-errno", os.path.join(ROOT, "oracle", "refcpu.cpp"),
+           "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    return out if r.returncode == 0 else None
+
+
+def cpu_baseline(W, aemh, target_s=12.0):
+    """The oracle on the same C2 workload on this host: the reference's
+    >= 50,000-index structure (per-thread grids and a merge, parallel G2P),
+    built -O3 -march=native -flto here, on the cores this process may use
+    (at most the box's CPU share of 16), over ~target_s seconds of steps
+    after one warm-up step."""
     from oracle import refcpu as R
+    path = native_oracle()
+    if path:
+        R.use_library(path)
     wl = W.c2_snow_dam()
     p, m, e = aemh.material_arrays(wl.materials)
     M = R.Materials(p, m, e)
     nx, ny, nz, dx, o = wl.grid
     try:
-        cores = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+        affinity = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    cores = max(1, min(affinity, share))
     cfg = R.make_config(nx, ny, nz, dx, o, 1, wl.gravity, 0.995, wl.bc_margin, 0.0, R.ACCUM_F32,
                         R.CLAMP_AUTO, threads=cores)
     st = R.State(wl.pos, mass=wl.mass, mat=wl.mat)
     R.step(cfg, M, st, wl.dt)  # warm-up
-    n_steps = 3
+    n_steps = 0
     t0 = time.perf_counter()
-    for _ in range(n_steps):
+    while n_steps < 5 or (time.perf_counter() - t0 < target_s and n_steps < 200):
         R.step(cfg, M, st, wl.dt)
+        n_steps += 1
     el = time.perf_counter() - t0
     return {"value": wl.n * n_steps / el, "unit": "particle-updates/s", "cores": cores,
             "kind": "port",
+            "host": {"cpu": cpu_model(), "nproc": os.cpu_count(), "affinity_cores": affinity,
+                     "cpu_share": share,
+                     "build": "g++ -O3 -march=native -flto -ffp-contract=off" if path
+                              else "prebuilt -O3 -march=x86-64-v3 (native build failed)"},
             "sample": f"oracle (C++ restatement, per-thread grid fold + merge like the "
                       f"reference's rayon path) on the same 1,048,576-particle 128^3 snow workload, "
-                      f"{n_steps} steps after 1 warm-up, {cores} threads"}
+                      f"steps 2-{n_steps + 1} ({el:.1f} s) after 1 warm-up, {cores} threads"}
 
 
-def extra_c3(W, aemh, steps=10):
+def extra_c3(W, aemh, steps=96):
+    """The configuration's 100 steps: 4 untimed (the first, the autotuner's
+    state, two more), then steps 5-100 timed; the profiled replay repeats
+    them from a snapshot."""
     wl = W.c3_three_boxes()
     eng = aemh.Engine()
     W.load_engine(eng, wl)
@@ -183,7 +229,7 @@ def extra_c3(W, aemh, steps=10):
     }
 
 
-def extra_c5(W, aemh, steps=20):
+def extra_c5(W, aemh, steps=498):
     """4,194,304 particles on 192^3, C and F - I stored in BF16 (FP32 compute),
     FP64 energy/momentum reductions every 10 steps inside the timed loop; the
     same run with FP32 storage beside it.  The same pair on the all-Neo-Hookean
@@ -213,6 +259,7 @@ def extra_c5(W, aemh, steps=20):
         kern = {k: eng.profile_get(k)[0] for k in KERNELS}
         eng.close()
         out[name] = {"particle_updates_per_s": wl.n * steps / el, "ms_per_step": el / steps * 1e3,
+                     "steps": f"3-{steps + 2} timed (2 untimed), FP64 sums every 10 inside",
                      "p2g_ms": round(kern["p2g"], 4), "g2p_ms": round(kern["g2p"], 4),
                      "fp64_sums_last": [float(x) for x in sums[-1]]}
     out["bf16_vs_f32_speedup"] = out["bf16"]["particle_updates_per_s"] / out["f32"]["particle_updates_per_s"]
@@ -222,11 +269,55 @@ def extra_c5(W, aemh, steps=20):
     return out
 
 
-def extra_batched(W, aemh, steps=5):
+def c5_error(W, aemh, n=1 << 16, res=48, steps=500):
+    """The C5 error report at its configured horizon: the mixed-precision
+    workload (snow box + Neo-Hookean box) at 65,536 particles on 48^3 (its
+    particle density), 500 steps on the GPU with C and F - I stored in BF16,
+    FP16 or FP32 (FP32 compute), against an all-FP64 run of the oracle on the
+    same seed: rel-L2 of x, v, x - x0 and F."""
+    import numpy as np
+    from oracle import refcpu as R
+    wl = W.c5_mixed_precision(n=n, res=res)
+    nx, ny, nz, dx, o = wl.grid
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    cfg = R.make_config(nx, ny, nz, dx, o, 1, wl.gravity, 0.995, wl.bc_margin, 0.0, R.ACCUM_F32,
+                        R.CLAMP_AUTO, threads=max(1, min(share, os.cpu_count() or 1)))
+    p, m, e = aemh.material_arrays(wl.materials)
+    M = R.Materials(p, m, e)
+    ref = R.State(wl.pos, mass=wl.mass, mat=wl.mat, dtype=np.float64)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        R.step(cfg, M, ref, wl.dt)
+    t_ref = time.perf_counter() - t0
+    x0 = wl.pos.astype(np.float64)
+
+    def rel(a, b):
+        a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    out = {"workload": f"c5_mixed scaled: {n} particles, {res}^3, snow + Neo-Hookean, {steps} steps, "
+                       f"against the FP64 oracle ({t_ref:.1f} s on the host)"}
+    for name, storage in (("bf16", aemh.STORE_BF16), ("f16", aemh.STORE_F16), ("f32", aemh.STORE_F32)):
+        wl.storage = storage
+        eng = aemh.Engine()
+        W.load_engine(eng, wl)
+        for _ in range(steps):
+            eng.step(wl.dt)
+        d = eng.download()
+        eng.close()
+        out[name] = {"x": rel(d["pos"], ref.pos), "v": rel(d["vel"], ref.vel),
+                     "x_minus_x0": rel(d["pos"] - x0, ref.pos - x0), "F": rel(d["F"], ref.F)}
+    return out
+
+
+def extra_batched(W, aemh, steps=96):
+    """The configuration's 100 steps: 4 untimed batched substeps (the first
+    runs the autotuner and builds the graphs), then 96 timed; the plain form
+    runs the same 4 + 96 per scene."""
     wl = W.c4_batched_scenes()
     eng = aemh.Engine()
     W.load_engine(eng, wl)
-    eng.batch_step(2)
+    eng.batch_step(4)
     t0 = time.perf_counter()
     eng.batch_step(steps)
     el_b = time.perf_counter() - t0
@@ -241,7 +332,8 @@ def extra_batched(W, aemh, steps=5):
         e.set_grid(sc.nx, sc.ny, sc.nz, sc.dx, tuple(sc.origin))
         e.set_materials(wl.materials)
         e.upload(wl.pos[sl], mass=wl.mass[sl], mat=wl.mat[sl])
-        e.step(sc.dt)
+        for _ in range(4):
+            e.step(sc.dt)
         engines.append((e, sc.dt))
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -453,7 +545,7 @@ def probe(W, aemh, which="c2"):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=195)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
@@ -510,9 +602,9 @@ def main():
     want = set() if args.no_extras else {e.strip() for e in args.extras.split(",") if e.strip()}
     if "q32" in want:
         log("[bench] deterministic Q32.32 mode")
-        wq, elq, elq_g, kq, _, _, _ = run_main(W, aemh, max(5, args.steps // 5), 2,
+        wq, elq, elq_g, kq, _, _, _ = run_main(W, aemh, args.steps, args.warmup,
                                                  accum=aemh.ACCUM_Q32_32)
-        extra["q32_particle_updates_per_s"] = wq.n * max(5, args.steps // 5) / elq
+        extra["q32_particle_updates_per_s"] = wq.n * args.steps / elq
     if "c3" in want:
         log("[bench] C3 16M particles")
         extra["c3"] = extra_c3(W, aemh)
@@ -522,6 +614,9 @@ def main():
     if "c5" in want:
         log("[bench] C5 mixed precision")
         extra["c5"] = extra_c5(W, aemh)
+        if not args.no_cpu:
+            log("[bench] C5 error against the FP64 oracle, 65,536 particles x 500 steps")
+            extra["c5"]["error_vs_fp64"] = c5_error(W, aemh)
     if "tiers" in want:
         log("[bench] LOD tier tick at engine scale")
         extra["tiers"] = extra_tiers(W, aemh, with_cpu=not args.no_cpu)

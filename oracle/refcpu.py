@@ -40,6 +40,14 @@ class RefMaterials(C.Structure):
 _lib = None
 
 
+def use_library(path):
+    """Load the oracle from another build of oracle/refcpu.cpp (bench.py's
+    CPU baseline builds one with -march=native -flto on the host that runs it)."""
+    global _lib, LIB_PATH
+    LIB_PATH = path
+    _lib = None
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE, "-B"], check=True)
 

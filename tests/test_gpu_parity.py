@@ -398,16 +398,19 @@ def test_graph_replays_with_eager_work_between():
 # FP32 mode
 # ---------------------------------------------------------------------------
 def test_c1_fp32_one_step_grid_and_particles():
-    """One substep: grid sums differ only by float summation order."""
+    """One substep.  The FP32 mode accumulates the momentum after the force
+    impulse, p + f dt, per contribution (its force plane reads zero); grid
+    sums differ from the oracle's by float summation order and that regrouping."""
     wl = W.c1_elastic_cube()
     eng = gpu_engine(wl)
     st = oracle_state(wl)
     g = R.step(oracle_cfg(wl), oracle_mats(wl.materials), st, wl.dt, want_grid=True)
     eng.step(wl.dt)
     gg = eng.download_grid(0, 32 ** 3)
+    impulse = g["p"].astype(np.float64) + g["f"].astype(np.float64) * np.float32(wl.dt)
     assert rel(gg["m"], g["m"]) < 1e-6
-    assert rel(gg["p"], g["p"]) < 1e-6
-    assert rel(gg["f"], g["f"]) < 1e-6
+    assert rel(gg["p"], impulse) < 1e-6
+    assert np.all(gg["f"] == 0)
     assert rel(gg["v"], g["v"]) < 1e-6
     d = eng.download()
     assert rel(d["pos"], st.pos) < 1e-7
