@@ -377,9 +377,23 @@ __device__ __forceinline__ void svd3(const float* F, float* U, float* sig, float
 // Snow plasticity after the F update (Stomakhin et al. 2013): singular values
 // outside [1 - theta_c, 1 + theta_s] are clamped, J_p *= sigma / sigma_hat,
 // F = U diag(sigma_hat) V^T.  F and J_p are left unchanged when nothing is
-// clamped, which the Gershgorin discs of F^T F (eigenvalues sigma^2) decide
-// without an SVD when they lie inside [lo^2, hi^2].  No reference counterpart
-// in the FP32 solver.
+// clamped, which two tests on C = F^T F (eigenvalues sigma^2) decide without
+// an SVD: its Gershgorin discs lying inside [lo^2, hi^2], or else C - lo^2 I
+// and hi^2 I - C both positive definite by Sylvester's criterion (every
+// leading minor > 0: every eigenvalue strictly inside).  The same float
+// operations as the oracle's.  No reference counterpart in the FP32 solver.
+__device__ __forceinline__ bool sym3_strictly_inside(float c00, float c11, float c22, float c01,
+                                                     float c02, float c12, float lo2, float hi2) {
+  const float a0 = c00 - lo2, a1 = c11 - lo2, a2 = c22 - lo2;
+  const float b0 = hi2 - c00, b1 = hi2 - c11, b2 = hi2 - c22;
+  const float da2 = a0 * a1 - c01 * c01;
+  const float db2 = b0 * b1 - c01 * c01;
+  const float da3 = a0 * (a1 * a2 - c12 * c12) - c01 * (c01 * a2 - c12 * c02) + c02 * (c01 * c12 - a1 * c02);
+  // hi^2 I - C has off-diagonals -c01, -c02, -c12
+  const float db3 = b0 * (b1 * b2 - c12 * c12) - c01 * (c01 * b2 + c12 * c02) - c02 * (c01 * c12 + b1 * c02);
+  return a0 > 0.0f && da2 > 0.0f && da3 > 0.0f && b0 > 0.0f && db2 > 0.0f && db3 > 0.0f;
+}
+
 __device__ __forceinline__ void snow_clamp(const MatDev& M, float* F, float* Jp,
                                            uint32_t* fp64_count = nullptr) {
   const float lo = 1.0f - M.theta_c, hi = 1.0f + M.theta_s;
@@ -387,14 +401,16 @@ __device__ __forceinline__ void snow_clamp(const MatDev& M, float* F, float* Jp,
     const float c00 = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
     const float c11 = F[1] * F[1] + F[4] * F[4] + F[7] * F[7];
     const float c22 = F[2] * F[2] + F[5] * F[5] + F[8] * F[8];
-    const float c01 = fabsf(F[0] * F[1] + F[3] * F[4] + F[6] * F[7]);
-    const float c02 = fabsf(F[0] * F[2] + F[3] * F[5] + F[6] * F[8]);
-    const float c12 = fabsf(F[1] * F[2] + F[4] * F[5] + F[7] * F[8]);
+    const float s01 = F[0] * F[1] + F[3] * F[4] + F[6] * F[7];
+    const float s02 = F[0] * F[2] + F[3] * F[5] + F[6] * F[8];
+    const float s12 = F[1] * F[2] + F[4] * F[5] + F[7] * F[8];
+    const float c01 = fabsf(s01), c02 = fabsf(s02), c12 = fabsf(s12);
     const float lo2 = lo * lo, hi2 = hi * hi;
     const float r0 = c01 + c02, r1 = c01 + c12, r2 = c02 + c12;
     if (c00 - r0 >= lo2 && c00 + r0 <= hi2 && c11 - r1 >= lo2 && c11 + r1 <= hi2 &&
         c22 - r2 >= lo2 && c22 + r2 <= hi2)
       return;
+    if (sym3_strictly_inside(c00, c11, c22, s01, s02, s12, lo2, hi2)) return;
   }
   float U[9], s[3], V[9];
   svd3(F, U, s, V, fp64_count);

@@ -593,9 +593,24 @@ int svd3_r(const double* F, double* U, double* sig, double* V) {
 // Snow plasticity clamp applied in G2P after F' = (I + dt grad v) F.
 // Stomakhin et al. 2013: sigma clamped to [1-theta_c, 1+theta_s];
 // J_p *= sigma/sigma_hat per axis; F <- U diag(sigma_hat) V^T.  F and J_p stay
-// unchanged when no singular value is clamped; the Gershgorin discs of F^T F
-// (whose eigenvalues are sigma^2) decide that without an SVD when they lie
-// inside [lo^2, hi^2].
+// unchanged when no singular value is clamped.  Two tests on C = F^T F
+// (whose eigenvalues are sigma^2) decide that without an SVD: its Gershgorin
+// discs lying inside [lo^2, hi^2], or else C - lo^2 I and hi^2 I - C both
+// positive definite by Sylvester's criterion (every leading minor > 0), which
+// holds exactly when every eigenvalue lies strictly inside (lo^2, hi^2).
+template <class R>
+bool sym3_strictly_inside(R c00, R c11, R c22, R c01, R c02, R c12, R lo2, R hi2) {
+  // det [[m0, x, y], [x, m1, z], [y, z, m2]] = m0 (m1 m2 - z^2) - x (x m2 - z y) + y (x z - m1 y)
+  R a0 = c00 - lo2, a1 = c11 - lo2, a2 = c22 - lo2;
+  R b0 = hi2 - c00, b1 = hi2 - c11, b2 = hi2 - c22;
+  R da2 = a0 * a1 - c01 * c01;
+  R db2 = b0 * b1 - c01 * c01;
+  R da3 = a0 * (a1 * a2 - c12 * c12) - c01 * (c01 * a2 - c12 * c02) + c02 * (c01 * c12 - a1 * c02);
+  // hi^2 I - C has off-diagonals -c01, -c02, -c12
+  R db3 = b0 * (b1 * b2 - c12 * c12) - c01 * (c01 * b2 + c12 * c02) - c02 * (c01 * c12 + b1 * c02);
+  return a0 > R(0) && da2 > R(0) && da3 > R(0) && b0 > R(0) && db2 > R(0) && db3 > R(0);
+}
+
 template <class R>
 void snow_clamp(const Mat<R>& m, R* F, R* Jp) {
   R lo = R(1.0) - m.theta_c, hi = R(1.0) + m.theta_s;
@@ -603,14 +618,16 @@ void snow_clamp(const Mat<R>& m, R* F, R* Jp) {
     R c00 = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
     R c11 = F[1] * F[1] + F[4] * F[4] + F[7] * F[7];
     R c22 = F[2] * F[2] + F[5] * F[5] + F[8] * F[8];
-    R c01 = fabs(F[0] * F[1] + F[3] * F[4] + F[6] * F[7]);
-    R c02 = fabs(F[0] * F[2] + F[3] * F[5] + F[6] * F[8]);
-    R c12 = fabs(F[1] * F[2] + F[4] * F[5] + F[7] * F[8]);
+    R s01 = F[0] * F[1] + F[3] * F[4] + F[6] * F[7];
+    R s02 = F[0] * F[2] + F[3] * F[5] + F[6] * F[8];
+    R s12 = F[1] * F[2] + F[4] * F[5] + F[7] * F[8];
+    R c01 = fabs(s01), c02 = fabs(s02), c12 = fabs(s12);
     R lo2 = lo * lo, hi2 = hi * hi;
     R r0 = c01 + c02, r1 = c01 + c12, r2 = c02 + c12;
     if (c00 - r0 >= lo2 && c00 + r0 <= hi2 && c11 - r1 >= lo2 && c11 + r1 <= hi2 &&
         c22 - r2 >= lo2 && c22 + r2 <= hi2)
       return;
+    if (sym3_strictly_inside(c00, c11, c22, s01, s02, s12, lo2, hi2)) return;
   }
   R U[9], s[3], V[9];
   svd3_r(F, U, s, V);
