@@ -66,54 +66,103 @@ struct P2GPart {
   float c[9], sv[9];
 };
 
-// The 27-node scatter in the reference's dk-dj-di order; `put(f, di, dj, dk,
-// gx, gy, gz, value)` receives each contribution of field f (m, p0..2,
-// f0..2; with kFold: m and p_r + f_r, the force term then being computed from
-// the dt-scaled stress the caller stored in sv).  Full MPM multiplies
-// (wx wy) wz, the velocity-only tier wx (wy wz).
+// The 27-node scatter; `put(f, di, dj, dk, gx, gy, gz, value)` receives each
+// contribution of field f (m, p0..2, f0..2; with kFold: m and p_r + f_r, the
+// force term then being computed from the dt-scaled stress the caller stored
+// in sv).  Each value is the reference's expression, evaluated in its order
+// (mpm_solver.rs:862-898): w = (wx wy) wz (velocity-only tier: wx (wy wz)),
+// u_r = ((v_r + C_r0 dx_g) + C_r1 dy_g) + C_r2 dz_g, m w u_r = (w m) u_r,
+// f_r = (sv_r0 N'_x + sv_r1 N'_y) + sv_r2 N'_z.  Their partial sums depend on
+// fewer than three axes, so the loops run di, dj, dk outermost to innermost
+// and each partial is formed once where its axes are fixed (a node's skip
+// test would otherwise keep the compiler from sharing them); a particle adds
+// one value per node, so the order in which its nodes are visited does not
+// change any node's sum.
 template <bool kFull, bool kFold = false, class Put>
 __device__ __forceinline__ void p2g_nodes(const int n0, const int n1, const int n2, const float dx,
                                           const float o0, const float o1, const float o2,
                                           const Stencil& st, const P2GPart& q, Put&& put) {
+  if (kFull) {
+    // per-axis terms: C_rc d_c and sv_rc N'_c for c = y, z (x is formed in the loop)
+    float cy[3][3], cz[3][3], sy[3][3], sz[3][3];
 #pragma unroll
-  for (int dk = 0; dk < 3; dk++)
+    for (int a = 0; a < 3; a++) {
+      const float dyg = ((float)(st.g[1] + a) + 0.5f) * dx + o1 - q.py;
+      const float dzg = ((float)(st.g[2] + a) + 0.5f) * dx + o2 - q.pz;
 #pragma unroll
-    for (int dj = 0; dj < 3; dj++)
+      for (int r = 0; r < 3; r++) {
+        cy[r][a] = q.c[3 * r + 1] * dyg;
+        cz[r][a] = q.c[3 * r + 2] * dzg;
+        sy[r][a] = q.sv[3 * r + 1] * q.dwy[a];
+        sz[r][a] = q.sv[3 * r + 2] * q.dwz[a];
+      }
+    }
+    const float vv[3] = {q.vx, q.vy, q.vz};
 #pragma unroll
-      for (int di = 0; di < 3; di++) {
-        const float w = kFull ? q.wx[di] * q.wy[dj] * q.wz[dk] : q.wx[di] * (q.wy[dj] * q.wz[dk]);
-        const int gx = st.g[0] + di, gy = st.g[1] + dj, gz = st.g[2] + dk;
-        if (w < 1e-10f || gx >= n0 || gy >= n1 || gz >= n2) continue;
-        if (kFull) {
-          const float dxg = ((float)gx + 0.5f) * dx + o0 - q.px;
-          const float dyg = ((float)gy + 0.5f) * dx + o1 - q.py;
-          const float dzg = ((float)gz + 0.5f) * dx + o2 - q.pz;
-          const float ux = q.vx + q.c[0] * dxg + q.c[1] * dyg + q.c[2] * dzg;
-          const float uy = q.vy + q.c[3] * dxg + q.c[4] * dyg + q.c[5] * dzg;
-          const float uz = q.vz + q.c[6] * dxg + q.c[7] * dyg + q.c[8] * dzg;
-          const float f0 = q.sv[0] * q.dwx[di] + q.sv[1] * q.dwy[dj] + q.sv[2] * q.dwz[dk];
-          const float f1 = q.sv[3] * q.dwx[di] + q.sv[4] * q.dwy[dj] + q.sv[5] * q.dwz[dk];
-          const float f2 = q.sv[6] * q.dwx[di] + q.sv[7] * q.dwy[dj] + q.sv[8] * q.dwz[dk];
-          put(0, di, dj, dk, gx, gy, gz, w * q.mass);
-          if (kFold) {
-            put(1, di, dj, dk, gx, gy, gz, w * q.mass * ux + f0);
-            put(2, di, dj, dk, gx, gy, gz, w * q.mass * uy + f1);
-            put(3, di, dj, dk, gx, gy, gz, w * q.mass * uz + f2);
-          } else {
-            put(1, di, dj, dk, gx, gy, gz, w * q.mass * ux);
-            put(2, di, dj, dk, gx, gy, gz, w * q.mass * uy);
-            put(3, di, dj, dk, gx, gy, gz, w * q.mass * uz);
-            put(4, di, dj, dk, gx, gy, gz, f0);
-            put(5, di, dj, dk, gx, gy, gz, f1);
-            put(6, di, dj, dk, gx, gy, gz, f2);
+    for (int di = 0; di < 3; di++) {
+      const int gx = st.g[0] + di;
+      const float dxg = ((float)gx + 0.5f) * dx + o0 - q.px;
+      float ux0[3], fx0[3];
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        ux0[r] = vv[r] + q.c[3 * r] * dxg;
+        fx0[r] = q.sv[3 * r] * q.dwx[di];
+      }
+#pragma unroll
+      for (int dj = 0; dj < 3; dj++) {
+        const int gy = st.g[1] + dj;
+        const float wxy = q.wx[di] * q.wy[dj];
+        float uxy[3], fxy[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          uxy[r] = ux0[r] + cy[r][dj];
+          fxy[r] = fx0[r] + sy[r][dj];
+        }
+#pragma unroll
+        for (int dk = 0; dk < 3; dk++) {
+          const int gz = st.g[2] + dk;
+          const float w = wxy * q.wz[dk];
+          if (w < 1e-10f || gx >= n0 || gy >= n1 || gz >= n2) continue;
+          const float wm = w * q.mass;
+          float u[3], f[3];
+#pragma unroll
+          for (int r = 0; r < 3; r++) {
+            u[r] = uxy[r] + cz[r][dk];
+            f[r] = fxy[r] + sz[r][dk];
           }
-        } else {
-          put(0, di, dj, dk, gx, gy, gz, w * q.mass);
-          put(1, di, dj, dk, gx, gy, gz, w * q.mass * q.vx);
-          put(2, di, dj, dk, gx, gy, gz, w * q.mass * q.vy);
-          put(3, di, dj, dk, gx, gy, gz, w * q.mass * q.vz);
+          put(0, di, dj, dk, gx, gy, gz, wm);
+          if (kFold) {
+            put(1, di, dj, dk, gx, gy, gz, wm * u[0] + f[0]);
+            put(2, di, dj, dk, gx, gy, gz, wm * u[1] + f[1]);
+            put(3, di, dj, dk, gx, gy, gz, wm * u[2] + f[2]);
+          } else {
+            put(1, di, dj, dk, gx, gy, gz, wm * u[0]);
+            put(2, di, dj, dk, gx, gy, gz, wm * u[1]);
+            put(3, di, dj, dk, gx, gy, gz, wm * u[2]);
+            put(4, di, dj, dk, gx, gy, gz, f[0]);
+            put(5, di, dj, dk, gx, gy, gz, f[1]);
+            put(6, di, dj, dk, gx, gy, gz, f[2]);
+          }
         }
       }
+    }
+  } else {
+#pragma unroll
+    for (int di = 0; di < 3; di++)
+#pragma unroll
+      for (int dj = 0; dj < 3; dj++)
+#pragma unroll
+        for (int dk = 0; dk < 3; dk++) {
+          const float w = q.wx[di] * (q.wy[dj] * q.wz[dk]);
+          const int gx = st.g[0] + di, gy = st.g[1] + dj, gz = st.g[2] + dk;
+          if (w < 1e-10f || gx >= n0 || gy >= n1 || gz >= n2) continue;
+          const float wm = w * q.mass;
+          put(0, di, dj, dk, gx, gy, gz, wm);
+          put(1, di, dj, dk, gx, gy, gz, wm * q.vx);
+          put(2, di, dj, dk, gx, gy, gz, wm * q.vy);
+          put(3, di, dj, dk, gx, gy, gz, wm * q.vz);
+        }
+  }
 }
 
 // The P2G inputs of working slot p: position, velocity, mass, the B-spline
