@@ -1237,7 +1237,11 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
   float fj;
   if (f_valid) {
     const MatDev& M = mats[mi < (uint32_t)nmat ? mi : (uint32_t)nmat];
+#if AEMH_DET
     if (M.model == kModelSnow) snow_clamp(M, nf, &jp, s.counters + kCntSvd64);
+#else
+    if (M.model == kModelSnow) snow_clamp_fast(M, nf, &jp, s.counters + kCntSvd64);
+#endif
     fj = det3(nf);
     if (fj < 0.1f) fj = 0.1f;
     if (fj > 10.0f) fj = 10.0f;

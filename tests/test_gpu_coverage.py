@@ -153,3 +153,60 @@ def test_serial_fp32_c3_three_boxes_100_steps_bitexact():
           f"v {rel(d['vel'], st.vel):.3e}")
     _assert_exact(d, st, "c3 step 100")
     eng.close()
+
+
+def _rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    return q if np.linalg.det(q) > 0 else -q
+
+
+def test_fp32_snow_clamp_eigen_route_matches_svd_route():
+    """The FP32 mode clamps snow through the eigen-decomposition of F^T F
+    when cond(F) <= 2 and det F > 0, and through the SVD otherwise; the
+    oracle always through the SVD.  One step of dt 1e-9 (F' = F to rounding,
+    so only the clamp differs) from snow F with singular values spread across
+    [0.85, 1.2] (at least 0.03 apart) or inside the window (eigen route),
+    with det F < 0 or cond(F) 2.5 to 11 (SVD route): F within 1e-6 and J_p
+    within 2e-6 (relative) of the oracle's, per particle; the SVD route's
+    particles bit for bit."""
+    n = 6000
+    wl = W.c2_snow_dam(n=n, steps=1, res=32)
+    rng = np.random.default_rng(77)
+    F = np.empty((n, 3, 3))
+    kind = np.arange(n) % 6
+    for i in range(n):
+        while True:
+            s = rng.uniform(0.85, 1.2, 3)
+            if np.min(np.diff(np.sort(s))) > 0.03:
+                break
+        if kind[i] == 0:
+            s = rng.uniform(0.98, 1.005, 3)
+        elif kind[i] == 1:
+            s[np.argmin(s)] *= -1.0
+        elif kind[i] == 2:
+            s = np.array([rng.uniform(0.1, 0.4), 1.0, 1.1])
+        F[i] = _rotation(rng) @ np.diag(s) @ _rotation(rng).T
+    F = F.reshape(n, 9).astype(np.float32)
+    J = np.ones(n, np.float32)
+    dt = 1e-9
+    eng = _engine(wl, aemh.ACCUM_F32)
+    eng.upload(wl.pos, F9=F, J=J, mass=wl.mass, mat=wl.mat)
+    st = R.State(wl.pos, F9=F, J=J, mass=wl.mass, mat=wl.mat)
+    eng.step(dt)
+    R.step(_oracle_cfg(wl, R.ACCUM_F32, 0), _mats(wl.materials), st, dt)
+    d = eng.download()
+    dF = np.abs(d["F"].reshape(n, 9) - st.F.reshape(n, 9)).max(axis=1)
+    dJp = np.abs(d["Jp"] / st.Jp - 1.0)
+    for k, name in ((0, "inside"), (1, "det < 0"), (2, "cond > 2"), (3, "spread")):
+        sel = kind == k if k < 3 else kind >= 3
+        print(f"{name:9s}: max |dF| {dF[sel].max():.3e}, max |dJp/Jp| {dJp[sel].max():.3e}, "
+              f"clamped {int(np.sum(st.Jp[sel] != 1.0))} of {int(sel.sum())}")
+    assert np.all(np.isfinite(d["F"])) and np.all(np.isfinite(d["Jp"]))
+    assert dF.max() <= 1e-6 and dJp.max() <= 2e-6, (dF.max(), dJp.max())
+    svd = (kind == 1) | (kind == 2)
+    assert np.array_equal(d["F"].reshape(n, 9)[svd], st.F.reshape(n, 9)[svd])
+    assert np.array_equal(d["Jp"][svd], st.Jp[svd])
+    # the routes did clamp: every spread, det < 0 and ill-conditioned F
+    assert np.all(st.Jp[kind >= 1] != 1.0)
+    eng.close()
