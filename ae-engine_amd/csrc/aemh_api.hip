@@ -551,6 +551,7 @@ void collect_profile(aemh_ctx* c) {
 template <typename Fn>
 aemh_status time_launches(aemh_ctx* c, int reps, float* ms, Fn&& fn) {
   hipEvent_t a = get_event(c), b = get_event(c);
+  fn();  // untimed: the first launch of a candidate pays cold caches and code loads
   HIPTRY(c, hipEventRecord(a, c->stream));
   for (int r = 0; r < reps; r++) fn();
   HIPTRY(c, hipEventRecord(b, c->stream));

@@ -17,23 +17,22 @@ namespace aemh {
 // Quadratic B-spline, ae_particle/src/mpm_solver.rs:442-464.  NaN input gives
 // 0 (both comparisons fail), as in the reference.
 // ---------------------------------------------------------------------------
+// Both branches are evaluated and selected: the same operations on the same
+// operands as the branching form, without the divergent branches that the
+// per-lane fractional positions would otherwise cause.
 __device__ __forceinline__ float b_spline(float x) {
-  float ax = fabsf(x);
-  if (ax < 0.5f) return 0.75f - ax * ax;
-  if (ax < 1.5f) {
-    float t = 1.5f - ax;
-    return 0.5f * t * t;
-  }
-  return 0.0f;
+  const float ax = fabsf(x);
+  const float inner = 0.75f - ax * ax;
+  const float t = 1.5f - ax;
+  const float outer = 0.5f * t * t;
+  return ax < 0.5f ? inner : (ax < 1.5f ? outer : 0.0f);
 }
 __device__ __forceinline__ float b_spline_deriv(float x) {
-  float ax = fabsf(x);
-  if (ax < 0.5f) return -2.0f * x;
-  if (ax < 1.5f) {
-    float sg = signbit(x) ? -1.0f : 1.0f;  // f32::signum (x is never NaN here)
-    return -(1.5f - ax) * sg;
-  }
-  return 0.0f;
+  const float ax = fabsf(x);
+  const float inner = -2.0f * x;
+  const float sg = signbit(x) ? -1.0f : 1.0f;  // f32::signum (x is never NaN here)
+  const float outer = -(1.5f - ax) * sg;
+  return ax < 0.5f ? inner : (ax < 1.5f ? outer : 0.0f);
 }
 
 // ---------------------------------------------------------------------------

@@ -59,6 +59,12 @@ __host__ __device__ constexpr int g2p_tiles_for(int wg) {
   return (2 * (wg >> 6)) < kG2PMaxTiles ? 2 * (wg >> 6) : kG2PMaxTiles;
 }
 
+// Compile-time switch passed to generic lambdas.
+template <bool B>
+struct BoolTag {
+  static constexpr bool value = B;
+};
+
 // Per-particle P2G inputs after the constitutive update.
 struct P2GPart {
   float wx[3], wy[3], wz[3], dwx[3], dwy[3], dwz[3];
@@ -778,6 +784,7 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
   __shared__ int gmeta[kG2PMaxTiles][8];  // origin x, y, z; grid n x, y, z; nb0; nb1
   __shared__ uint32_t gboff[kG2PMaxTiles];
   __shared__ uint32_t wcnt[kG2PMaxWG / 64];
+  __shared__ int gnonfinite;  // some staged node carries a non-finite velocity
   const int max_tiles = g2p_tiles_for((int)blockDim.x);
   const int t = threadIdx.x;
   const int lane = __lane_id(), wave = t >> 6;
@@ -829,6 +836,7 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
   const Stencil st = make_stencil(S, px, py, pz);
   const uint32_t key = valid ? block_key(S, st) : 0xffffffffu;
   skey[t] = key;
+  if (t == 0) gnonfinite = 0;
   __syncthreads();
   const bool head = valid && (t == 0 || skey[t - 1] != key);
   const unsigned long long hb = __ballot(head);
@@ -869,6 +877,7 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
         node_axes(gboff[ti], gmeta[ti][6], gmeta[ti][7], x, y, z, ux, uy, uz);
         v = G.gv[ux + uy + uz];
         if (v.w < 1e-12f) v.x = v.y = v.z = 0.0f;
+        if (!(isfinite(v.x) && isfinite(v.y) && isfinite(v.z))) gnonfinite = 1;
       }
       gtile[ti * kTileNodes + li] = v;
     }
@@ -1070,20 +1079,38 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
       zg[r] += t[r] * dwzk;
     }
   };
-  if (use_lds) {
-    // Staged tile: node (di, dj, dk) sits at a constant offset from the
-    // particle's base node.  Nodes past the grid's far faces get weight 0
-    // (an axis weight, so w0 = 0 < 1e-10 skips them as the in-grid test
-    // does); nodes with m < 1e-12 were staged with v = 0.  Only the weight
-    // test is left per node: a skipped node's u is a selected +0 (not a
-    // product with a zero weight, which a non-finite grid velocity would
-    // turn into NaN), the same bits as the reference skipping it.
-    float wxm[3], wym[3];
+  // Staged tile: node (di, dj, dk) sits at a constant offset from the
+  // particle's base node.  Nodes past the grid's far faces get weight 0 (an
+  // axis weight, so w0 = 0 < 1e-10 skips them as the in-grid test does);
+  // nodes with m < 1e-12 were staged with v = 0.
+  float wxm[3], wym[3], wzm[3];
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-      wxm[a] = ix[a] ? wx[a] : 0.0f;
-      wym[a] = iy[a] ? wy[a] : 0.0f;
-    }
+  for (int a = 0; a < 3; a++) {
+    wxm[a] = ix[a] ? wx[a] : 0.0f;
+    wym[a] = iy[a] ? wy[a] : 0.0f;
+    wzm[a] = iz[a] ? wz[a] : 0.0f;
+  }
+  // The weight test decides nothing for a particle whose node weights are
+  // all either 0 or >= 1e-10: a zero weight times a finite staged velocity
+  // adds +-0, which leaves every sum's bits as the reference's `continue`
+  // does (the accumulators start at +0).  The smallest non-zero w0 =
+  // (wx wy) wz is the product of each axis's smallest non-zero weight
+  // (rounded products are monotone), so one test per particle covers its 27
+  // nodes; a wave takes the test-free loop when all its LDS-path lanes pass
+  // and no staged node is non-finite.
+  bool tiny = false;
+  if (use_lds) {
+    constexpr float kBig = 3.0e38f;
+    float mw[3];
+    const float* ws[3] = {wxm, wym, wzm};
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      mw[c] = fminf(fminf(ws[c][0] > 0.0f ? ws[c][0] : kBig, ws[c][1] > 0.0f ? ws[c][1] : kBig),
+                    ws[c][2] > 0.0f ? ws[c][2] : kBig);
+    tiny = !((mw[0] * mw[1]) * mw[2] >= 1e-10f);
+  }
+  const bool test_free = !__any(tiny) && !gnonfinite;
+  if (use_lds) {
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -1095,34 +1122,48 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
     const float4* tb = gtile + slot * kTileNodes + ox + kTileSide * oy + kTileSide * kTileSide * oz;
     // Per row of 3 nodes (fixed dj, dk) the row sum feeds the y-axis sums
     // and the layer sum once, instead of every node adding to all three.
+    // With the weight test, a skipped node's u is a selected +0 (not a
+    // product with a zero weight, which a non-finite grid velocity would
+    // turn into NaN), the same bits as the reference skipping it.
+    auto lds_gather = [&](auto tag) {
+      constexpr bool kTest = decltype(tag)::value;
 #pragma unroll 1
-    for (int dk = 0; dk < 3; dk++) {
-      const float wzk = iz[dk] ? wz[dk] : 0.0f;
-      const float4* tk = tb + dk * (kTileSide * kTileSide);
-      float t[3];
+      for (int dk = 0; dk < 3; dk++) {
+        const float wzk = wzm[dk];
+        const float4* tk = tb + dk * (kTileSide * kTileSide);
+        float t[3];
 #pragma unroll
-      for (int dj = 0; dj < 3; dj++) {
-        float row[3];
+        for (int dj = 0; dj < 3; dj++) {
+          float row[3];
 #pragma unroll
-        for (int di = 0; di < 3; di++) {
-          const float w0 = wxy[dj][di] * wzk;
-          const float4 g = tk[di + kTileSide * dj];
-          const bool keep = !(w0 < 1e-10f);
-          const float u[3] = {keep ? w0 * g.x : 0.0f, keep ? w0 * g.y : 0.0f, keep ? w0 * g.z : 0.0f};
+          for (int di = 0; di < 3; di++) {
+            const float w0 = wxy[dj][di] * wzk;
+            const float4 g = tk[di + kTileSide * dj];
+            float u[3] = {w0 * g.x, w0 * g.y, w0 * g.z};
+            if (kTest) {
+              const bool keep = !(w0 < 1e-10f);
+#pragma unroll
+              for (int r = 0; r < 3; r++) u[r] = keep ? u[r] : 0.0f;
+            }
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+              sx[di][r] += u[r];
+              row[r] = di == 0 ? u[r] : row[r] + u[r];
+            }
+          }
 #pragma unroll
           for (int r = 0; r < 3; r++) {
-            sx[di][r] += u[r];
-            row[r] = di == 0 ? u[r] : row[r] + u[r];
+            sy[dj][r] += row[r];
+            t[r] = dj == 0 ? row[r] : t[r] + row[r];
           }
         }
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-          sy[dj][r] += row[r];
-          t[r] = dj == 0 ? row[r] : t[r] + row[r];
-        }
+        layer_sums(dk, t);
       }
-      layer_sums(dk, t);
-    }
+    };
+    if (test_free)
+      lds_gather(BoolTag<false>{});
+    else
+      lds_gather(BoolTag<true>{});
   } else {
 #pragma unroll 1
     for (int dk = 0; dk < 3; dk++) {
