@@ -50,10 +50,17 @@ BYTES = {"p2g": (110, 28), "g2p": (156, 16), "grid_update": (0, 44), "step": (21
 KERNELS = ["count", "scan", "scatter", "p2g", "grid_update", "g2p"]
 # HBM bytes per dispatch from rocprofv3 FETCH_SIZE (x 2.000) and WRITE_SIZE
 # (x 1.000), calibrated against a 256 MiB copy, as recorded in
-# profiles/c2_snow_1m_hbm_traffic.md (1M-particle snow workload) and
-# profiles/c3_three_boxes_16m_g2p.md (16M particles, in-place G2P).
-TRAFFIC_C2 = {"p2g": 246.3e6, "g2p": 251.1e6, "grid_update": 142.7e6}
-TRAFFIC_C3 = {"p2g": 3017e6, "g2p": 3068e6}
+# profiles/p2g_g2p_sq_counters.md (1M-particle snow workload, launch shape
+# 4,1,256,2048; 16M-particle workload, early steps, 4,0,256,8192).
+TRAFFIC_C2 = {"p2g": 191.5e6, "g2p": 224.8e6, "grid_update": 90.7e6}
+TRAFFIC_C3 = {"p2g": 2637e6, "g2p": 3068e6}
+# VALU instructions per wave (one wave = 64 particles) from rocprofv3
+# SQ_INSTS_VALU / SQ_WAVES, recorded in profiles/p2g_g2p_sq_counters.md.  The
+# issue ceiling without packed FP32: one wave64 VALU instruction per 4 cycles
+# per SIMD, 4 SIMDs x 256 CUs at 2.4 GHz = 614.4e9 wave-instructions/s.
+VALU_PEAK_WINST = 256 * 4 * 2.4e9 / 4
+VALU_PER_WAVE_C2 = {"p2g": 1806, "g2p": 1421}
+VALU_PER_WAVE_C3 = {"p2g": 1831, "g2p": 1295}
 
 
 def log(msg):
@@ -112,15 +119,25 @@ def run_main(W, aemh, steps, warmup, accum=None):
     return wl, el, el_prof, kern, blocks, npart, shape
 
 
-def roofline_of(kern, name, npart, blocks, traffic=None):
+def roofline_of(kern, name, npart, blocks, traffic=None, valu=None):
+    """HBM roofline of one kernel: algorithmic bytes per launch over its average
+    HIP-event duration.  With `valu` (VALU instructions per wave from the
+    counter profile), also the VALU issue rate against its ceiling: P2G is
+    limited by VALU issue and LDS atomics long before HBM."""
     per_p, per_n = BYTES[name]
     nbytes = npart * per_p + blocks * 64 * per_n
     ms = kern[name]
     achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None if traffic is None else int(traffic.get(name, 0)) or None,
-            "kernel": name, "algorithmic_bytes": int(nbytes), "avg_ms": round(ms, 4)}
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4),
+           "traffic": None if traffic is None else int(traffic.get(name, 0)) or None,
+           "kernel": name, "algorithmic_bytes": int(nbytes), "avg_ms": round(ms, 4)}
+    if valu and name in valu and ms > 0:
+        rate = (npart + 63) // 64 * valu[name] / (ms * 1e-3)
+        out["valu_issue"] = {"insts_per_wave": valu[name], "achieved": round(rate / 1e9, 1),
+                             "peak": VALU_PEAK_WINST / 1e9, "unit": "G wave-instructions/s",
+                             "frac": round(rate / VALU_PEAK_WINST, 4)}
+    return out
 
 
 def cpu_model():
@@ -208,9 +225,22 @@ def extra_c3(W, aemh, steps=96):
     restore(eng, wl, snap, shape)
     eng.step(wl.dt)
     eng.profile(True)
-    eng.profile_reset()
-    timed_steps(eng, wl.dt, steps)
-    kern = {k: eng.profile_get(k)[0] for k in KERNELS}
+    # per-step kernel times of the replayed window, averaged over the whole
+    # window and over windows of 10 steps (P2G and G2P grow as the boxes land:
+    # the snow starts to yield and fast fluid leaves its sorted order)
+    per = {k: [] for k in KERNELS}
+    for _ in range(steps):
+        eng.profile_reset()
+        eng.step(wl.dt)
+        for k in KERNELS:
+            per[k].append(eng.profile_get(k)[0])
+    kern = {k: sum(v) / len(v) for k, v in per.items()}
+    first = 101 - steps
+    windows = {}
+    for a in range(0, steps, 10):
+        b = min(a + 10, steps)
+        windows[f"steps {first + a}-{first + b - 1}"] = {
+            k: round(sum(per[k][a:b]) / (b - a), 4) for k in ("p2g", "g2p", "count")}
     blocks, npart = eng.last_substep_info()
     eng.close()
     total_blocks = 64 ** 3
@@ -222,8 +252,9 @@ def extra_c3(W, aemh, steps=96):
         "ms_per_step": el / steps * 1e3,
         "active_block_fraction": blocks / total_blocks,
         "step_hbm_frac": round(step_bytes / (el / steps) / 1e9 / HBM_PEAK_GBS, 4),
-        "g2p": roofline_of(kern, "g2p", npart, blocks, TRAFFIC_C3),
-        "p2g": roofline_of(kern, "p2g", npart, blocks, TRAFFIC_C3),
+        "g2p": roofline_of(kern, "g2p", npart, blocks, TRAFFIC_C3, VALU_PER_WAVE_C3),
+        "p2g": roofline_of(kern, "p2g", npart, blocks, TRAFFIC_C3, VALU_PER_WAVE_C3),
+        "ms_by_window": windows,
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
         "launch_shape": shape,
     }
@@ -509,13 +540,15 @@ def tier_cpu_port(tw, aemh, lod, near, mid, far, dt):
                       "the tick cadences (classify 1/8, near 1/1, mid 1/12, lifetimes 2/1)"}
 
 
-def probe(W, aemh, which="c2"):
+def probe(W, aemh, which="c2", shape=None):
     """Counter-collection run.  The copy moves exactly 256 MiB in and 256 MiB
     out, so FETCH_SIZE / WRITE_SIZE of its dispatch give the KiB each counter
     reports per real KiB; the dispatches of the 5 steps after the autotuner
     (the launch shape the benchmark uses) follow, on the 1M-particle snow
-    workload (c2) or the 16M-particle three-box workload (c3); or 10 steps of
-    the 1M-particle FixedPoint cloud (fixed)."""
+    workload (c2) or the 16M-particle three-box workload (c3); c3late runs
+    steps 2-96 of the 16M workload instead, so its last dispatches are late
+    steps (yielding snow, splashed fluid); or 10 steps of the 1M-particle
+    FixedPoint cloud (fixed)."""
     import torch
     x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
@@ -531,13 +564,19 @@ def probe(W, aemh, which="c2"):
         sim.close()
         print("probe done: fixed cloud, 10 steps", flush=True)
         return
-    wl = W.c3_three_boxes() if which == "c3" else W.c2_snow_dam()
+    wl = W.c3_three_boxes() if which in ("c3", "c3late") else W.c2_snow_dam()
     eng = aemh.Engine()
     W.load_engine(eng, wl)
     eng.step(wl.dt)
-    shape = eng.autotune()
-    for _ in range(5):
+    if shape:
+        eng.set_launch_shape(*shape)
+        shape = eng.launch_shape()
+    else:
+        shape = eng.autotune()
+    for s in range(95 if which == "c3late" else 5):
         eng.step(wl.dt)
+        if s % 10 == 9:
+            print("probe step", s + 2, flush=True)
     eng.close()
     print("probe done, launch shape", shape, flush=True)
 
@@ -552,11 +591,16 @@ def main():
     ap.add_argument("--extras", default="q32,c3,c4,c5,tiers,fixed",
                     help="comma list of the extra workloads to run (q32, c3, c4, c5, tiers, "
                          "fixed)")
-    ap.add_argument("--probe", nargs="?", const="c2", choices=("c2", "c3", "fixed"),
+    ap.add_argument("--probe", nargs="?", const="c2", choices=("c2", "c3", "c3late", "fixed"),
                     help="short run for hardware-counter collection: a 256 MiB device copy "
                          "(calibrates FETCH_SIZE / WRITE_SIZE) then 5 steps of the main workload "
-                         "(c2, the default) or of the 16M-particle workload (c3), or 10 steps of "
-                         "the FixedPoint cloud (fixed)")
+                         "(c2, the default) or of the 16M-particle workload (c3), steps 2-96 of "
+                         "the 16M-particle workload (c3late: the last 5 are its late steps), or "
+                         "10 steps of the FixedPoint cloud (fixed)")
+    ap.add_argument("--shape", default=None,
+                    help="with --probe: a fixed launch shape 'p2g_tiles,p2g_partials,g2p_wg,"
+                         "upd_wgs' instead of the autotuner's pick (the autotuner's timings move "
+                         "under a profiler)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the MPM step runs on a single GPU and does not "
@@ -566,7 +610,8 @@ def main():
     from aemh import workloads as W
 
     if args.probe:
-        probe(W, aemh, args.probe)
+        probe(W, aemh, args.probe,
+              tuple(int(x) for x in args.shape.split(",")) if args.shape else None)
         return
 
     log("[bench] main workload: C2 1M snow particles, 128^3")
@@ -592,12 +637,12 @@ def main():
                    "particles": wl.n, "grid": [128, 128, 128], "accum": "f32", "substeps": 1,
                    "launch_shape": {k: shape[k] for k in ("p2g_tiles", "p2g_partials", "g2p_wg",
                                                             "upd_wgs")}},
-        "roofline": roofline_of(kern, dominant, npart, blocks, TRAFFIC_C2),
+        "roofline": roofline_of(kern, dominant, npart, blocks, TRAFFIC_C2, VALU_PER_WAVE_C2),
         "step_hbm_frac": round(step_bytes / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
     }
     extra = {"profiled_window_particle_updates_per_s": wl.n * args.steps / el_prof,
-             "g2p_roofline": roofline_of(kern, "g2p", npart, blocks, TRAFFIC_C2),
+             "g2p_roofline": roofline_of(kern, "g2p", npart, blocks, TRAFFIC_C2, VALU_PER_WAVE_C2),
              "active_blocks": blocks}
     want = set() if args.no_extras else {e.strip() for e in args.extras.split(",") if e.strip()}
     if "q32" in want:

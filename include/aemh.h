@@ -283,7 +283,10 @@ aemh_status aemh_set_launch_shape(aemh_ctx* ctx, uint32_t p2g_tiles, uint32_t p2
 /* Grid readback for parity checks (reference layout, x-fastest index
  * i + j nx + k nx ny, mpm_solver.rs:157-159) of scene `scene`, as left by
  * the last substep: after P2G (mass, momentum, force) and after the update
- * (velocity).  NULL outputs are skipped. */
+ * (velocity).  NULL outputs are skipped.  The FP32 mode (AEMH_ACCUM_F32)
+ * accumulates the force into the momentum as its impulse: momentum3 then
+ * holds p + f dt and force3 is zero.  The Q32.32 and reference-order modes
+ * return m, p and f apart. */
 aemh_status aemh_download_grid(aemh_ctx* ctx, uint32_t scene, float* mass,
                                float* momentum3, float* force3, float* vel3);
 /* Q32.32 raw sums (7 per node: m, p0..2, f0..2) of the last substep. */
