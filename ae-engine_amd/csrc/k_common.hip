@@ -109,6 +109,116 @@ __global__ __launch_bounds__(256) void k_count_lds(PBuf cur, const SceneDev* __r
   }
 }
 
+// Grids with more node blocks than k_count_lds bins.  A nearly block-sorted
+// working set (re-sorted, or in place a few steps after a re-sort) gives
+// each workgroup's 1024 keys a short range, so they are counted in LDS bins
+// at key - kmin and each touched bin takes one HBM atomic: a few returning
+// atomics per workgroup instead of one per run of every wave, whose
+// latency bounds k_count.  A workgroup whose keys span more than
+// kCountWinBins blocks counts with k_count's per-wave run atomics.
+constexpr uint32_t kCountWinBins = 2048;
+
+__global__ __launch_bounds__(256) void k_count_win(PBuf cur, const SceneDev* __restrict__ sc,
+                                                   int ns, uint32_t n, SortDev s) {
+  __shared__ uint32_t bins[kCountWinBins];
+  __shared__ uint32_t wlo[4], whi[4];
+  const int lane = __lane_id(), wave = (int)(threadIdx.x >> 6);
+  const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  uint32_t keys[kCountRounds];
+  uint32_t lo = 0xffffffffu, hi = 0u;
+#pragma unroll
+  for (int r = 0; r < kCountRounds; r++) {
+    const uint32_t i = (blockIdx.x * kCountRounds + r) * 256u + threadIdx.x;
+    uint32_t key = 0xffffffffu;
+    if (i < n) {
+      const SceneDev S = sc[scene_of(sc, ns, i)];
+      const Stencil st = make_stencil(S, cur.x[0][i], cur.x[1][i], cur.x[2][i]);
+      key = block_key(S, st);
+      s.key[i] = key;
+      lo = min(lo, key);
+      hi = max(hi, key);
+    }
+    keys[r] = key;
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    lo = min(lo, (uint32_t)__shfl_xor((int)lo, d, 64));
+    hi = max(hi, (uint32_t)__shfl_xor((int)hi, d, 64));
+  }
+  if (lane == 0) {
+    wlo[wave] = lo;
+    whi[wave] = hi;
+  }
+  __syncthreads();
+  const uint32_t kmin = min(min(wlo[0], wlo[1]), min(wlo[2], wlo[3]));
+  const uint32_t kmax = max(max(whi[0], whi[1]), max(whi[2], whi[3]));
+  // The workgroup's first particle is below n, so kmin <= kmax.
+  if (kmax - kmin >= kCountWinBins) {
+    uint32_t nruns = 0;
+#pragma unroll
+    for (int r = 0; r < kCountRounds; r++) {
+      const uint32_t i = (blockIdx.x * kCountRounds + r) * 256u + threadIdx.x;
+      const bool valid = i < n;
+      const uint32_t key = keys[r];
+      const uint32_t prev = __shfl_up(key, 1, 64);
+      const bool head = valid && (lane == 0 || prev != key);
+      const unsigned long long heads = __ballot(head);
+      const int nvalid = __popcll(__ballot(valid));
+      nruns += (uint32_t)__popcll(heads);
+      uint32_t old = 0;
+      if (head) {
+        const unsigned long long above = heads & ~le;
+        const int next = above ? (__ffsll((long long)above) - 1) : nvalid;
+        old = atomicAdd(&s.cnt[key], (uint32_t)(next - lane));
+      }
+      const unsigned long long mine = valid ? (heads & le) : 1ull;
+      const int h = 63 - __clzll((long long)mine);
+      const uint32_t base = __shfl(old, h, 64);
+      if (valid) s.rank[i] = base + (uint32_t)(lane - h);
+    }
+    if (lane == 0) count_runs(s, blockIdx.x * 4u + (uint32_t)wave, nruns);
+    return;  // the whole workgroup takes this branch
+  }
+  for (uint32_t b = threadIdx.x; b <= kmax - kmin; b += 256u) bins[b] = 0u;
+  __syncthreads();
+  uint32_t loc[kCountRounds];
+  bool own[kCountRounds];
+  uint32_t nruns = 0;
+#pragma unroll
+  for (int r = 0; r < kCountRounds; r++) {
+    const uint32_t i = (blockIdx.x * kCountRounds + r) * 256u + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t key = keys[r];
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool head = valid && (lane == 0 || prev != key);
+    const unsigned long long heads = __ballot(head);
+    const int nvalid = __popcll(__ballot(valid));
+    nruns += (uint32_t)__popcll(heads);
+    uint32_t old = 0;
+    if (head) {
+      const unsigned long long above = heads & ~le;
+      const int next = above ? (__ffsll((long long)above) - 1) : nvalid;
+      old = atomicAdd(&bins[key - kmin], (uint32_t)(next - lane));
+    }
+    const unsigned long long mine = valid ? (heads & le) : 1ull;
+    const int h = 63 - __clzll((long long)mine);
+    const uint32_t base = __shfl(old, h, 64);
+    loc[r] = base + (uint32_t)(lane - h);
+    own[r] = head && old == 0u;  // first adder of this bin in the workgroup
+  }
+  if (lane == 0) count_runs(s, blockIdx.x * 4u + (uint32_t)wave, nruns);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kCountRounds; r++)
+    if (own[r]) bins[keys[r] - kmin] = atomicAdd(&s.cnt[keys[r]], bins[keys[r] - kmin]);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kCountRounds; r++) {
+    const uint32_t i = (blockIdx.x * kCountRounds + r) * 256u + threadIdx.x;
+    if (i < n) s.rank[i] = bins[keys[r] - kmin] + loc[r];
+  }
+}
+
 // Every non-empty base block activates the 2x2x2 node blocks its 6^3 halo
 // can touch (idempotent plain stores, no atomics).
 __device__ __forceinline__ void activate_halo(const SceneDev* __restrict__ sc, int ns, SortDev& s,
@@ -389,7 +499,7 @@ void launch_count(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, uint32_t
   if (nb_total <= kCountLdsBins)
     launch_k(k_count_lds, grid_for(n, 256 * kCountRounds), 256, 0, st, cur, sc, ns, n, s, nb_total);
   else
-    launch_k(k_count, grid_for(n, 256), 256, 0, st, cur, sc, ns, n, s);
+    launch_k(k_count_win, grid_for(n, 256 * kCountRounds), 256, 0, st, cur, sc, ns, n, s);
 }
 void launch_scan(hipStream_t st, const SceneDev* sc, int ns, SortDev s, uint32_t nb) {
   const uint32_t tiles = (nb + kScanTile - 1) / kScanTile;
