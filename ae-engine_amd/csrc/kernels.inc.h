@@ -78,7 +78,9 @@ struct P2GPart {
 // in sv).  Each value is the reference's expression, evaluated in its order
 // (mpm_solver.rs:862-898): w = (wx wy) wz (velocity-only tier: wx (wy wz)),
 // u_r = ((v_r + C_r0 dx_g) + C_r1 dy_g) + C_r2 dz_g, m w u_r = (w m) u_r,
-// f_r = (sv_r0 N'_x + sv_r1 N'_y) + sv_r2 N'_z.  Their partial sums depend on
+// f_r = (sv_r0 N'_x + sv_r1 N'_y) + sv_r2 N'_z; with kFold (the FP32 mode,
+// whose sums have no fixed order anyway) p_r + f_r is one fused
+// multiply-add, (w m) u_r + f_r rounded once.  Their partial sums depend on
 // fewer than three axes, so the loops run di, dj, dk outermost to innermost
 // and each partial is formed once where its axes are fixed (a node's skip
 // test would otherwise keep the compiler from sharing them); a particle adds
@@ -138,9 +140,12 @@ __device__ __forceinline__ void p2g_nodes(const int n0, const int n1, const int 
           }
           put(0, di, dj, dk, gx, gy, gz, wm);
           if (kFold) {
-            put(1, di, dj, dk, gx, gy, gz, wm * u[0] + f[0]);
-            put(2, di, dj, dk, gx, gy, gz, wm * u[1] + f[1]);
-            put(3, di, dj, dk, gx, gy, gz, wm * u[2] + f[2]);
+            // The FP32 mode's sums have no fixed order (the reference's own
+            // >= 50,000-index path), so its momentum-plus-impulse value is
+            // one fused multiply-add: a rounding step less per contribution.
+            put(1, di, dj, dk, gx, gy, gz, __builtin_fmaf(wm, u[0], f[0]));
+            put(2, di, dj, dk, gx, gy, gz, __builtin_fmaf(wm, u[1], f[1]));
+            put(3, di, dj, dk, gx, gy, gz, __builtin_fmaf(wm, u[2], f[2]));
           } else {
             put(1, di, dj, dk, gx, gy, gz, wm * u[0]);
             put(2, di, dj, dk, gx, gy, gz, wm * u[1]);
@@ -237,8 +242,8 @@ __device__ __forceinline__ long long rint_i64(double y) {
 // rint_i64(v * scale) for a power-of-two scale with |v scale| < 2^51, in
 // three VALU: the product is exact, so one fused multiply-add with the same
 // constant rounds exactly as the separate product and sum do.
-__device__ __forceinline__ long long rint_i64_scaled(float v, double scale) {
-  const double y = __builtin_fma((double)v, scale, 6755399441055744.0);
+__device__ __forceinline__ long long rint_i64_scaled(float v, double scale, double magic) {
+  const double y = __builtin_fma((double)v, scale, magic);
   return (long long)((unsigned long long)__double_as_longlong(y) - 0x4338000000000000ull);
 }
 
@@ -504,9 +509,14 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
                   (unsigned long long)a);
       };
 #else
+      // 1.5 2^52, held in registers: as a literal, the compiler rebuilds it
+      // with two moves at every node after a fused multiply-add overwrote it.
+      // The empty asm emits nothing.
+      double magic = 6755399441055744.0;
+      asm volatile("" : "+v"(magic));
       auto put = [&](int f, int di, int dj, int dk, int, int, int, float v) {
         const int g = f == 0 ? 0 : 1;
-        const long long a = rint_i64_scaled(v, scale[g]);  // |v scale| < 2^50
+        const long long a = rint_i64_scaled(v, scale[g], magic);  // |v scale| < 2^50
         atomicAdd((unsigned long long*)&tile[tb + f * kTileNodes + (dk * kTileSide + dj) * kTileSide + di],
                   (unsigned long long)a);
       };
