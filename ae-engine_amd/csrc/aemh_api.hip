@@ -40,9 +40,11 @@ const char* kVersion = "aemh 0.1 (gfx950)";
 constexpr uint32_t kResortMax = 64;
 // When both copies of the working set fit in MI355X's 256 MB Infinity Cache
 // the re-sorting G2P's extra traffic (mass, material, id, perm) stays in
-// that cache, and every substep re-sorts; above it, in-place steps save HBM
-// traffic.
+// that cache, and every kResortResident-th substep re-sorts (on the 1M-particle
+// snow workload, every 2nd: step kernels 0.207 against 0.209-0.217 ms re-sorting
+// every substep, three runs each); above it, in-place steps save HBM traffic.
 constexpr size_t kInfinityCacheBytes = (size_t)256 << 20;
+constexpr uint32_t kResortResident = 2;
 // Working-set slots are addressed with 32-bit byte offsets in the kernels.
 constexpr uint32_t kMaxWork = 1u << 30;
 
@@ -820,9 +822,10 @@ aemh_status run_substeps(aemh_ctx* c, uint32_t k, bool allow_graph = true) {
     // Re-sort on the first substep of a new working set, when the block
     // order has decayed, and at least every kResortMax substeps.
     const uint32_t every = c->cfg.resort_interval;
+    const bool resident = 2 * store_bytes(c->n_work, c->work[0].storage) <= kInfinityCacheBytes;
     const bool resort = every ? (c->need_sort || c->since_sort + 1 >= every)
-                              : (c->need_sort || c->since_sort + 1 >= kResortMax ||
-                                 2 * store_bytes(c->n_work, c->work[0].storage) <= kInfinityCacheBytes);
+                              : (c->need_sort ||
+                                 c->since_sort + 1 >= (resident ? kResortResident : kResortMax));
     counted_sorted = c->fresh_sort;
     if (graph) {
       hipGraphExec_t& ge = c->gexec[c->cur * 2 + (resort ? 1 : 0)];
