@@ -1463,15 +1463,18 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   CHECK_LAUNCH(c);
   constexpr int kReps = 3;
   float best = 1e30f, ms = 0.0f;
-  int pick = ls.p2g_tiles, pick_part = ls.p2g_partials;
+  int pick = ls.p2g_tiles, pick_part = ls.p2g_partials, pick_pairs = ls.p2g_pairs;
   // Tile budget and flush mode together: the flush mode moves work between
-  // P2G and the grid update, so both are timed.
+  // P2G and the grid update, so both are timed.  The FP32 mode also times
+  // the pair reduction.
   s = ensure_partials(c);
   if (s != AEMH_OK) return s;
+  for (int pairs = 0; pairs <= (q32 ? 0 : 1); pairs++)
   for (int part = 0; part <= 1; part++)
     for (int tiles = 2; tiles <= 4; tiles++) {
       ls.p2g_tiles = tiles;
       ls.p2g_partials = part;
+      ls.p2g_pairs = pairs;
       s = time_launches(c, kReps, &ms, [&] {
         launch_clear(st, c->sort, c->grid, acc_layout(c), ls.upd_wgs);
         if (q32) {
@@ -1483,10 +1486,11 @@ aemh_status aemh_autotune(aemh_ctx* c) {
         }
       });
       if (s != AEMH_OK) return s;
-      if (ms < best) best = ms, pick = tiles, pick_part = part;
+      if (ms < best) best = ms, pick = tiles, pick_part = part, pick_pairs = pairs;
     }
   ls.p2g_tiles = pick;
   ls.p2g_partials = pick_part;
+  ls.p2g_pairs = pick_pairs;
   c->tune_ms[0] = best;
   // Partials in the chosen tile layout for the grid-update candidates.
   launch_clear(st, c->sort, c->grid, acc_layout(c), ls.upd_wgs);
@@ -1538,7 +1542,7 @@ aemh_status aemh_get_launch_shape(aemh_ctx* c, uint32_t* p2g_tiles, uint32_t* p2
                                   uint32_t* g2p_wg, uint32_t* upd_wgs, float* tuned_ms3) {
   if (!c) return AEMH_ERR_ARG;
   if (p2g_tiles) *p2g_tiles = (uint32_t)c->shape.p2g_tiles;
-  if (p2g_partials) *p2g_partials = (uint32_t)c->shape.p2g_partials;
+  if (p2g_partials) *p2g_partials = (uint32_t)(c->shape.p2g_partials | (c->shape.p2g_pairs << 1));
   if (g2p_wg) *g2p_wg = (uint32_t)c->shape.g2p_wg;
   if (upd_wgs) *upd_wgs = (uint32_t)c->shape.upd_wgs;
   if (tuned_ms3)
@@ -1550,12 +1554,14 @@ aemh_status aemh_set_launch_shape(aemh_ctx* c, uint32_t p2g_tiles, uint32_t p2g_
                                   uint32_t g2p_wg, uint32_t upd_wgs) {
   if (!c) return AEMH_ERR_ARG;
   if (p2g_tiles < 2 || p2g_tiles > 4) return fail(c, AEMH_ERR_ARG, "p2g_tiles must be 2, 3 or 4");
-  if (p2g_partials > 1) return fail(c, AEMH_ERR_ARG, "p2g_partials must be 0 or 1");
+  if (p2g_partials > 3)
+    return fail(c, AEMH_ERR_ARG, "p2g_partials must be 0-3 (bit 0 partial flush, bit 1 pairs)");
   if (g2p_wg != 64 && g2p_wg != 128 && g2p_wg != 256 && g2p_wg != 512)
     return fail(c, AEMH_ERR_ARG, "g2p_wg must be 64, 128, 256 or 512");
   if (upd_wgs < 1 || upd_wgs > 65536) return fail(c, AEMH_ERR_ARG, "upd_wgs must be in [1, 65536]");
   c->shape.p2g_tiles = (int)p2g_tiles;
-  c->shape.p2g_partials = (int)p2g_partials;
+  c->shape.p2g_partials = (int)(p2g_partials & 1u);
+  c->shape.p2g_pairs = (int)(p2g_partials >> 1);
   c->shape.g2p_wg = (int)g2p_wg;
   c->shape.upd_wgs = (int)upd_wgs;
   c->tuned = true;

@@ -41,6 +41,8 @@ struct LaunchShape {
   int p2g_tiles = 3;     // LDS tiles per P2G workgroup: 2, 3 or 4
   int p2g_partials = 0;  // P2G tile flush: 0 one HBM atomic per node value,
                          // 1 partial stores gathered by the grid update
+  int p2g_pairs = 0;     // FP32 mode: 1 pairs lanes of one base cell and adds
+                         // their node values across the pair before the LDS atomics
   int g2p_wg = 256;      // G2P workgroup size: 64, 128, 256 or 512
   int upd_wgs = 2048;    // persistent grid-update / clear workgroups
 };

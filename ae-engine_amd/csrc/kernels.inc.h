@@ -50,6 +50,8 @@ constexpr int kTileVals = kTileFields * kTileNodes;
 constexpr int kPartStride = AEMH_DET ? 8 : 4;
 // Lane layers of the P2G re-assignment (rank within a base cell, capped).
 constexpr int kLayers = 16;
+// Pair layers of the pair-reduced P2G (up to 8 pairs per base cell).
+constexpr int kPairLayers = 8;
 // Base-block tiles of grid (v, m) a G2P workgroup stages in LDS.
 // G2P stages up to two 6^3 grid tiles per wave of the workgroup in LDS
 // (dynamic shared memory, 3,456 B per tile), workgroups of up to 512.
@@ -176,6 +178,81 @@ __device__ __forceinline__ void p2g_nodes(const int n0, const int n1, const int 
   }
 }
 
+// Value of a float from the other lane of this lane's pair (2i <-> 2i + 1):
+// a DPP quad permutation [1, 0, 3, 2], every lane of the wave active.
+__device__ __forceinline__ float pair_swap(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+
+// The 27-node scatter of a pair of lanes whose particles share a base cell
+// (the FP32 mode's 4 values per node: m and p_r + f_r dt, the same values as
+// p2g_nodes<true, true>).  At each node lane 2i keeps fields 0 and 2 of its
+// particle and hands fields 1 and 3 to lane 2i + 1, which keeps those and
+// hands back 0 and 2; `put2(di, dj, dk, a, b)` then receives the pair's sums
+// of the lane's two fields (lo: m and p_y; hi: p_x and p_z).  A node that one
+// particle skips (w < 1e-10, outside the grid) adds 0 from that lane, as the
+// reference's `continue`; both lanes visit every node, so the lane swaps
+// always have their partner active.
+template <class Put2>
+__device__ __forceinline__ void p2g_nodes_pair(const int n0, const int n1, const int n2, const float dx,
+                                               const float o0, const float o1, const float o2,
+                                               const Stencil& st, const P2GPart& q, const bool hi,
+                                               Put2&& put2) {
+  float cy[3][3], cz[3][3], sy[3][3], sz[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float dyg = ((float)(st.g[1] + a) + 0.5f) * dx + o1 - q.py;
+    const float dzg = ((float)(st.g[2] + a) + 0.5f) * dx + o2 - q.pz;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      cy[r][a] = q.c[3 * r + 1] * dyg;
+      cz[r][a] = q.c[3 * r + 2] * dzg;
+      sy[r][a] = q.sv[3 * r + 1] * q.dwy[a];
+      sz[r][a] = q.sv[3 * r + 2] * q.dwz[a];
+    }
+  }
+  const float vv[3] = {q.vx, q.vy, q.vz};
+#pragma unroll
+  for (int di = 0; di < 3; di++) {
+    const int gx = st.g[0] + di;
+    const float dxg = ((float)gx + 0.5f) * dx + o0 - q.px;
+    float ux0[3], fx0[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      ux0[r] = vv[r] + q.c[3 * r] * dxg;
+      fx0[r] = q.sv[3 * r] * q.dwx[di];
+    }
+#pragma unroll
+    for (int dj = 0; dj < 3; dj++) {
+      const int gy = st.g[1] + dj;
+      const float wxy = q.wx[di] * q.wy[dj];
+      float uxy[3], fxy[3];
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        uxy[r] = ux0[r] + cy[r][dj];
+        fxy[r] = fx0[r] + sy[r][dj];
+      }
+#pragma unroll
+      for (int dk = 0; dk < 3; dk++) {
+        const int gz = st.g[2] + dk;
+        const float w = wxy * q.wz[dk];
+        const bool keep = !(w < 1e-10f || gx >= n0 || gy >= n1 || gz >= n2);
+        const float wm = w * q.mass;
+        float val[4];
+        val[0] = wm;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+          val[1 + r] = __builtin_fmaf(wm, uxy[r] + cz[r][dk], fxy[r] + sz[r][dk]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) val[k] = keep ? val[k] : 0.0f;
+        const float k0 = hi ? val[1] : val[0], k1 = hi ? val[3] : val[2];
+        const float s0 = hi ? val[0] : val[1], s1 = hi ? val[2] : val[3];
+        put2(di, dj, dk, k0 + pair_swap(s0), k1 + pair_swap(s1));
+      }
+    }
+  }
+}
+
 // The P2G inputs of working slot p: position, velocity, mass, the B-spline
 // weights and derivatives of its stencil and, in full MPM, C and the
 // volume-scaled stress of its material (material_table.get(i), else entry
@@ -285,11 +362,13 @@ __device__ __forceinline__ void tile_node_of(int t, int& lx, int& ly, int& lz) {
 // 2^-50 of the group's largest contribution.  Tiles are flushed block-
 // coalesced with one HBM atomic per non-zero node value.  The LDS and HBM
 // scatter paths are separate code so the LDS one stays ds_add.
-template <int STORE, int TILES>
+template <int STORE, int TILES, bool PAIRS = false>
 __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict__ sc, int ns,
                                              const MatDev* __restrict__ mats, int nmat,
                                              uint32_t n, SortDev s, GridDev G, int partials) {
   __shared__ long long tile[TILES * kTileVals];
+  // pair mode: lane index of each (cell, pair layer)'s first member
+  __shared__ uint8_t pbase[PAIRS ? TILES * 64 * kPairLayers : 1];
   __shared__ uint32_t skey[256];
   __shared__ uint32_t sperm[256];
   __shared__ uint32_t ccnt[TILES * 64];
@@ -361,18 +440,64 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
     // ---- layers: rank of the particle among the workgroup's particles of its
     // base cell.  Lanes are re-assigned layer by layer, so the 64 lanes of one
     // LDS atomic hit distinct nodes (the first kLayers - 1 layers). ----
-    int layer = kLayers - 1;
-    if (valid && slot < TILES) {
-      int c = 0;
+    if constexpr (!PAIRS) {
+      int layer = kLayers - 1;
+      if (valid && slot < TILES) {
+        int c = 0;
 #pragma unroll
-      for (int a = 2; a >= 0; a--) c = c * 4 + min(max(st.g[a] - torg[slot][a], 0), 3);
-      layer = (int)min(atomicAdd(&ccnt[slot * 64 + c], 1u), (uint32_t)(kLayers - 1));
+        for (int a = 2; a >= 0; a--) c = c * 4 + min(max(st.g[a] - torg[slot][a], 0), 3);
+        layer = (int)min(atomicAdd(&ccnt[slot * 64 + c], 1u), (uint32_t)(kLayers - 1));
+      }
+      const uint32_t rl = atomicAdd(&lcnt[layer], 1u);
+      __syncthreads();
+      uint32_t off = 0;
+      for (int l = 0; l < layer; l++) off += lcnt[l];
+      order[off + rl] = (uint8_t)t;
+    } else {
+      // ---- pair layers: two particles of one base cell go to lanes 2i and
+      // 2i + 1, which add their node values across lanes before the LDS
+      // atomics.  Pair layer L holds each cell's (L+1)-th pair, so the pairs of
+      // one LDS atomic still hit distinct nodes; lanes [0, npairs) are pairs,
+      // then the particles left unpaired, then those not staged in a tile.
+      // A particle pairs only if its base lies inside its tile's block (a
+      // base past the grid's far face is clamped into the block's cells). ----
+      int cell = -1;
+      uint32_t rank = 0;
+      if (valid && slot < TILES) {
+        int c = 0;
+        bool inside = true;
+#pragma unroll
+        for (int a = 2; a >= 0; a--) {
+          const int o = st.g[a] - torg[slot][a];
+          inside = inside && o >= 0 && o <= 3;
+          c = c * 4 + min(max(o, 0), 3);
+        }
+        if (inside) {
+          cell = slot * 64 + c;
+          rank = atomicAdd(&ccnt[cell], 1u);
+        }
+      }
+      __syncthreads();
+      // class 0: pair member (layer pl, member rank & 1); 1: unpaired; 2: no tile
+      int cls = (valid && slot < TILES) ? 1 : 2, pl = 0;
+      if (cell >= 0 && rank < (ccnt[cell] & ~1u) && rank < 2u * kPairLayers) {
+        cls = 0;
+        pl = (int)(rank >> 1);
+      }
+      uint32_t idx = 0;
+      if (cls == 0 && (rank & 1u) == 0u) {
+        idx = atomicAdd(&lcnt[pl], 2u);
+        pbase[cell * kPairLayers + pl] = (uint8_t)idx;
+      } else if (cls != 0) {
+        idx = atomicAdd(&lcnt[kPairLayers + cls - 1], 1u);
+      }
+      __syncthreads();
+      if (cls == 0 && (rank & 1u)) idx = (uint32_t)pbase[cell * kPairLayers + pl] + 1u;
+      const int li = cls == 0 ? pl : kPairLayers + cls - 1;
+      uint32_t off = 0;
+      for (int l = 0; l < li; l++) off += lcnt[l];
+      order[off + idx] = (uint8_t)t;
     }
-    const uint32_t rl = atomicAdd(&lcnt[layer], 1u);
-    __syncthreads();
-    uint32_t off = 0;
-    for (int l = 0; l < layer; l++) off += lcnt[l];
-    order[off + rl] = (uint8_t)t;
     for (int e = t; e < ntiles * kTileVals; e += 256) tile[e] = 0;
     __syncthreads();
   }
@@ -483,6 +608,17 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
   }
 #endif
 
+#if !AEMH_DET
+  // A wave of pair lanes scatters pair sums when every particle in it is a
+  // finite full-MPM one (others go down the per-particle paths).
+  bool pair_wave = false;
+  if constexpr (PAIRS) {
+    uint32_t npair = 0;
+#pragma unroll
+    for (int l = 0; l < kPairLayers; l++) npair += lcnt[l];
+    if ((uint32_t)(wave * 64 + 64) <= npair) pair_wave = __all(finite_in && full) != 0;
+  }
+#endif
   if (valid) {
     const int n0 = S.n[0], n1 = S.n[1], n2 = S.n[2];
     const float dx = S.dx, o0 = S.o[0], o1 = S.o[1], o2 = S.o[2];
@@ -496,6 +632,13 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
       const int tb = slot * kTileVals +
                      ((st.g[2] - torg[slot][2]) * kTileSide + (st.g[1] - torg[slot][1])) * kTileSide +
                      (st.g[0] - torg[slot][0]);
+#if !AEMH_DET
+      // 1.5 2^52, held in registers: as a literal, the compiler rebuilds it
+      // with two moves at every node after a fused multiply-add overwrote it.
+      // The empty asm emits nothing.
+      double magic = 6755399441055744.0;
+      asm volatile("" : "+v"(magic));
+#endif
 #if AEMH_DET
       unsigned long long magn[27];
 #pragma unroll
@@ -509,11 +652,6 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
                   (unsigned long long)a);
       };
 #else
-      // 1.5 2^52, held in registers: as a literal, the compiler rebuilds it
-      // with two moves at every node after a fused multiply-add overwrote it.
-      // The empty asm emits nothing.
-      double magic = 6755399441055744.0;
-      asm volatile("" : "+v"(magic));
       auto put = [&](int f, int di, int dj, int dk, int, int, int, float v) {
         const int g = f == 0 ? 0 : 1;
         const long long a = rint_i64_scaled(v, scale[g], magic);  // |v scale| < 2^50
@@ -521,10 +659,32 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
                   (unsigned long long)a);
       };
 #endif
+#if AEMH_DET
       if (full)
         p2g_nodes<true, kFoldForce>(n0, n1, n2, dx, o0, o1, o2, st, q, put);
       else
         p2g_nodes<false>(n0, n1, n2, dx, o0, o1, o2, st, q, put);
+#else
+      if (pair_wave) {
+        // Lane 2i (lo) adds the pair's mass and momentum-y sums, lane 2i + 1
+        // its momentum-x and momentum-z sums.
+        const bool hi = (t & 1) != 0;
+        const double sc0 = hi ? scale[1] : scale[0];
+        const int f0 = (hi ? 1 : 0) * kTileNodes, f1 = (hi ? 3 : 2) * kTileNodes;
+        auto put2 = [&](int di, int dj, int dk, float v0, float v1) {
+          const int node = tb + (dk * kTileSide + dj) * kTileSide + di;
+          atomicAdd((unsigned long long*)&tile[node + f0],
+                    (unsigned long long)rint_i64_scaled(v0, sc0, magic));  // |v scale| < 2^50
+          atomicAdd((unsigned long long*)&tile[node + f1],
+                    (unsigned long long)rint_i64_scaled(v1, scale[1], magic));
+        };
+        p2g_nodes_pair(n0, n1, n2, dx, o0, o1, o2, st, q, hi, put2);
+      } else if (full) {
+        p2g_nodes<true, kFoldForce>(n0, n1, n2, dx, o0, o1, o2, st, q, put);
+      } else {
+        p2g_nodes<false>(n0, n1, n2, dx, o0, o1, o2, st, q, put);
+      }
+#endif
 #if AEMH_DET
       // Sum-|c| bound per node in units of 2^32 raw, rounded up.
 #pragma unroll
@@ -1397,28 +1557,40 @@ void launch_p2g_serial(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, con
 }
 #endif
 
-template <int STORE>
+template <int STORE, bool PAIRS>
 void launch_p2g_store(hipStream_t st, int tiles, int partials, unsigned grid, PBuf cur,
                       const SceneDev* sc, int ns, const MatDev* mats, int nmat, uint32_t n,
                       SortDev s, GridDev g) {
   if (tiles <= 2)
-    launch_k(k_p2g<STORE, 2>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
+    launch_k(k_p2g<STORE, 2, PAIRS>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
   else if (tiles == 3)
-    launch_k(k_p2g<STORE, 3>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
+    launch_k(k_p2g<STORE, 3, PAIRS>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
   else
-    launch_k(k_p2g<STORE, 4>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
+    launch_k(k_p2g<STORE, 4, PAIRS>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
+}
+
+template <bool PAIRS>
+void launch_p2g_pairs(hipStream_t st, const LaunchShape& ls, PBuf cur, const SceneDev* sc, int ns,
+                      const MatDev* mats, int nmat, uint32_t n, SortDev s, GridDev g, int storage) {
+  const unsigned grid = (n + 255u) / 256u;
+  if (storage == 1)
+    launch_p2g_store<1, PAIRS>(st, ls.p2g_tiles, ls.p2g_partials, grid, cur, sc, ns, mats, nmat, n, s, g);
+  else if (storage == 2)
+    launch_p2g_store<2, PAIRS>(st, ls.p2g_tiles, ls.p2g_partials, grid, cur, sc, ns, mats, nmat, n, s, g);
+  else
+    launch_p2g_store<0, PAIRS>(st, ls.p2g_tiles, ls.p2g_partials, grid, cur, sc, ns, mats, nmat, n, s, g);
 }
 
 void launch_p2g(hipStream_t st, const LaunchShape& ls, PBuf cur, const SceneDev* sc, int ns,
                 const MatDev* mats, int nmat, uint32_t n, SortDev s, GridDev g, int storage) {
   if (!n) return;
-  const unsigned grid = (n + 255u) / 256u;
-  if (storage == 1)
-    launch_p2g_store<1>(st, ls.p2g_tiles, ls.p2g_partials, grid, cur, sc, ns, mats, nmat, n, s, g);
-  else if (storage == 2)
-    launch_p2g_store<2>(st, ls.p2g_tiles, ls.p2g_partials, grid, cur, sc, ns, mats, nmat, n, s, g);
-  else
-    launch_p2g_store<0>(st, ls.p2g_tiles, ls.p2g_partials, grid, cur, sc, ns, mats, nmat, n, s, g);
+#if !AEMH_DET
+  if (ls.p2g_pairs) {
+    launch_p2g_pairs<true>(st, ls, cur, sc, ns, mats, nmat, n, s, g, storage);
+    return;
+  }
+#endif
+  launch_p2g_pairs<false>(st, ls, cur, sc, ns, mats, nmat, n, s, g, storage);
 }
 
 void launch_grid_update(hipStream_t st, const LaunchShape& ls, const SceneDev* sc, int ns,

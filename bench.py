@@ -86,7 +86,7 @@ def restore(eng, wl, snap, shape):
     eng.upload(snap["pos"], vel=snap["vel"], C9=snap["C"], F9=snap["F"], J=snap["J"],
                mass=wl.mass, mat=wl.mat, Jp=snap["Jp"])
     eng.set_launch_shape(shape["p2g_tiles"], shape["p2g_partials"], shape["g2p_wg"],
-                         shape["upd_wgs"])
+                         shape["upd_wgs"], shape["p2g_pairs"])
 
 
 def run_main(W, aemh, steps, warmup, accum=None):
@@ -599,8 +599,8 @@ def main():
                          "10 steps of the FixedPoint cloud (fixed)")
     ap.add_argument("--shape", default=None,
                     help="with --probe: a fixed launch shape 'p2g_tiles,p2g_partials,g2p_wg,"
-                         "upd_wgs' instead of the autotuner's pick (the autotuner's timings move "
-                         "under a profiler)")
+                         "upd_wgs[,p2g_pairs]' instead of the autotuner's pick (the autotuner's "
+                         "timings move under a profiler)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the MPM step runs on a single GPU and does not "
@@ -635,8 +635,8 @@ def main():
         "config": {"workload": "c2_snow_1m_128: 1,048,576 snow particles in a dam-break block, "
                                "128^3 grid, dx 1/128, dt 5e-5, FP32 P2G",
                    "particles": wl.n, "grid": [128, 128, 128], "accum": "f32", "substeps": 1,
-                   "launch_shape": {k: shape[k] for k in ("p2g_tiles", "p2g_partials", "g2p_wg",
-                                                            "upd_wgs")}},
+                   "launch_shape": {k: shape[k] for k in ("p2g_tiles", "p2g_partials", "p2g_pairs",
+                                                            "g2p_wg", "upd_wgs")}},
         "roofline": roofline_of(kern, dominant, npart, blocks, TRAFFIC_C2, VALU_PER_WAVE_C2),
         "step_hbm_frac": round(step_bytes / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},

@@ -114,7 +114,7 @@ def test_profiled_steps_are_exact_and_timed():
 
 def test_launch_shape_rejects_bad_values():
     eng = aemh.Engine()
-    for bad in [(1, 1, 256, 2048), (5, 1, 256, 2048), (3, 2, 256, 2048), (3, 1, 96, 2048),
+    for bad in [(1, 1, 256, 2048), (5, 1, 256, 2048), (3, 4, 256, 2048), (3, 1, 96, 2048),
                 (3, 1, 1024, 2048), (3, 1, 256, 0)]:
         with pytest.raises(aemh.AemhError):
             eng.set_launch_shape(*bad)
@@ -122,20 +122,24 @@ def test_launch_shape_rejects_bad_values():
 
 
 def test_fp32_flush_modes_agree():
-    """FP32 mode: the atomic and the partial-sum tile flush differ only in
-    the order of the final per-node float additions."""
+    """FP32 mode: the atomic and the partial-sum tile flush, each with and
+    without the pair reduction (two particles of a base cell adding their
+    node values across lanes before the LDS atomics), differ only in the
+    order and grouping of float additions."""
     wl = W.c2_snow_dam(n=1 << 16, steps=10, res=64)
     out = []
-    for part in (0, 1):
+    for part, pairs in ((0, 0), (1, 0), (0, 1), (1, 1)):
         eng = aemh.Engine()
         W.load_engine(eng, wl)
-        eng.set_launch_shape(3, part, 256, 2048)
+        eng.set_launch_shape(3, part, 256, 2048, pairs)
+        assert eng.launch_shape()["p2g_pairs"] == pairs
         for _ in range(10):
             eng.step(wl.dt)
         out.append(eng.download())
         eng.close()
-    for f in ("pos", "vel", "F"):
-        assert rel(out[0][f], out[1][f]) < 1e-6, f
+    for k in (1, 2, 3):
+        for f in ("pos", "vel", "F"):
+            assert rel(out[0][f], out[k][f]) < 1e-6, (k, f)
 
 
 def _c5_scaled(storage, n=1 << 16, res=48):
