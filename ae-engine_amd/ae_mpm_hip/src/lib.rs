@@ -235,6 +235,8 @@ pub mod ffi {
         pub fn aemh_mpm_step_parallel_with_grid(ctx: *mut AemhCtx, idx: *const u32, n_idx: u32,
                                                 total_dt: f32) -> c_int;
         pub fn aemh_step_velocity_only(ctx: *mut AemhCtx, total_dt: f32) -> c_int;
+        pub fn aemh_step_async(ctx: *mut AemhCtx, total_dt: f32) -> c_int;
+        pub fn aemh_sync(ctx: *mut AemhCtx) -> c_int;
         pub fn aemh_set_scenes(ctx: *mut AemhCtx, scenes: *const AemhScene, n: u32) -> c_int;
         pub fn aemh_batch_step(ctx: *mut AemhCtx, n_substeps: u32) -> c_int;
         pub fn aemh_autotune(ctx: *mut AemhCtx) -> c_int;

@@ -146,7 +146,8 @@ EXPORTS = [
     "aemh_set_config", "aemh_set_grid", "aemh_set_materials", "aemh_upload_particles",
     "aemh_upload_plastic_j", "aemh_download_particles", "aemh_download_plastic_j",
     "aemh_set_indices", "aemh_step", "aemh_mpm_step_parallel_with_grid",
-    "aemh_step_velocity_only", "aemh_set_scenes", "aemh_batch_step", "aemh_autotune",
+    "aemh_step_velocity_only", "aemh_step_async", "aemh_sync", "aemh_set_scenes",
+    "aemh_batch_step", "aemh_autotune",
     "aemh_get_launch_shape", "aemh_set_launch_shape",
     "aemh_download_grid", "aemh_download_grid_q32", "aemh_conservation_f64",
     "aemh_conservation_f64_noalloc", "aemh_svd_fp64_count",
@@ -199,6 +200,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.aemh_step.argtypes = [P, C.c_float]
     lib.aemh_mpm_step_parallel_with_grid.argtypes = [P, C.POINTER(u32), u32, C.c_float]
     lib.aemh_step_velocity_only.argtypes = [P, C.c_float]
+    lib.aemh_step_async.argtypes = [P, C.c_float]
+    lib.aemh_sync.argtypes = [P]
     lib.aemh_set_scenes.argtypes = [P, C.POINTER(Scene), u32]
     lib.aemh_batch_step.argtypes = [P, u32]
     lib.aemh_autotune.argtypes = [P]
@@ -412,6 +415,13 @@ class Engine:
     # -- steps ----------------------------------------------------------------
     def step(self, total_dt: float):
         self._check(self._lib.aemh_step(self._h, C.c_float(total_dt)))
+
+    def step_async(self, total_dt: float):
+        """aemh_step without waiting for the GPU (sync() waits)."""
+        self._check(self._lib.aemh_step_async(self._h, C.c_float(total_dt)))
+
+    def sync(self):
+        self._check(self._lib.aemh_sync(self._h))
 
     def mpm_step_parallel_with_grid(self, particle_indices, total_dt: float):
         idx = _c(particle_indices, np.uint32)

@@ -159,6 +159,7 @@ struct aemh_ctx {
   uint32_t since_sort = 0;
   bool need_sort = true;
   bool fresh_sort = false;     // the last substep re-sorted
+  bool unsynced = false;       // substeps enqueued by aemh_step_async, not yet synchronized
   uint32_t runs_base = 0;      // wave runs counted right after a re-sort
   uint32_t* runs_h = nullptr;  // host-mapped: wave runs of the last count
   bool use_graph = true;
@@ -799,7 +800,21 @@ aemh_status check_overflow(aemh_ctx* c) {
   return AEMH_OK;
 }
 
-aemh_status run_substeps(aemh_ctx* c, uint32_t k, bool allow_graph = true) {
+// Reads the wave runs the last count left in host-mapped memory (the stream
+// is synchronized) and decides whether the next substep re-sorts.
+void update_sort_policy(aemh_ctx* c, bool counted_sorted) {
+  const uint32_t runs = *(volatile uint32_t*)c->runs_h;
+  if (counted_sorted)
+    c->runs_base = runs;
+  else if (c->runs_base && runs > c->runs_base + c->runs_base / 4)
+    c->need_sort = true;
+}
+
+// Enqueues k substeps; with sync, waits for them and reports the Q32.32
+// range check.  Without (aemh_step_async), the run-count re-sort trigger
+// waits for the next synchronized call and the range check for aemh_sync
+// or the next synchronous step (the overflow flag is sticky).
+aemh_status run_substeps(aemh_ctx* c, uint32_t k, bool allow_graph = true, bool sync = true) {
   if (!c->mats_d) return fail(c, AEMH_ERR_STATE, "materials not set");
   if (c->n_work == 0) return AEMH_OK;
   const bool serial = c->cfg.accum == AEMH_ACCUM_F32_SERIAL;
@@ -848,12 +863,13 @@ aemh_status run_substeps(aemh_ctx* c, uint32_t k, bool allow_graph = true) {
     c->fresh_sort = resort;
   }
   c->work_dirty = true;
+  if (!sync && !c->prof) {
+    c->unsynced = true;
+    return AEMH_OK;
+  }
   HIPTRY(c, hipStreamSynchronize(c->stream));
-  const uint32_t runs = *(volatile uint32_t*)c->runs_h;
-  if (counted_sorted)
-    c->runs_base = runs;
-  else if (c->runs_base && runs > c->runs_base + c->runs_base / 4)
-    c->need_sort = true;
+  c->unsynced = false;
+  update_sort_policy(c, counted_sorted);
   if (c->prof) collect_profile(c);
   return check_overflow(c);
 }
@@ -908,7 +924,7 @@ aemh_status prepare_single_scene(aemh_ctx* c, float sub_dt, int vel_only) {
   return set_scene_table(c, {sc});
 }
 
-aemh_status single_scene(aemh_ctx* c, float total_dt, int vel_only) {
+aemh_status single_scene(aemh_ctx* c, float total_dt, int vel_only, bool sync = true) {
   if (c->cfg.substeps == 0) return fail(c, AEMH_ERR_ARG, "substeps must be >= 1");
   if (!c->grid_set) return fail(c, AEMH_ERR_STATE, "grid not set");
   aemh_status s = ensure_work(c);
@@ -917,7 +933,7 @@ aemh_status single_scene(aemh_ctx* c, float total_dt, int vel_only) {
   s = prepare_single_scene(c, total_dt / (float)c->cfg.substeps,  // mpm_solver.rs:1075
                                        vel_only);
   if (s != AEMH_OK) return s;
-  return run_substeps(c, c->cfg.substeps);
+  return run_substeps(c, c->cfg.substeps, true, sync);
 }
 
 }  // namespace
@@ -1357,6 +1373,25 @@ aemh_status aemh_step(aemh_ctx* c, float total_dt) {
   if (s != AEMH_OK || !c->mpss) return s;
   uint32_t n_exp = 0;  // mpm_step_with_grid ends with update_lifetimes (:508)
   return aemh_mpss_update_lifetimes(c, total_dt, nullptr, 0, &n_exp);
+}
+
+aemh_status aemh_step_async(aemh_ctx* c, float total_dt) {
+  if (!c) return AEMH_ERR_ARG;
+  // A resident MpssBuffer ages its lifetimes after the step, which reads the
+  // expiry count back: that step stays synchronous.
+  if (c->mpss) return aemh_step(c, total_dt);
+  hipSetDevice(c->device);
+  return single_scene(c, total_dt, 0, false);
+}
+
+aemh_status aemh_sync(aemh_ctx* c) {
+  if (!c) return AEMH_ERR_ARG;
+  hipSetDevice(c->device);
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  if (!c->unsynced) return AEMH_OK;
+  c->unsynced = false;
+  update_sort_policy(c, c->fresh_sort);
+  return check_overflow(c);
 }
 
 aemh_status aemh_mpm_step_parallel_with_grid(aemh_ctx* c, const uint32_t* idx, uint32_t n_idx,

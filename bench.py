@@ -10,16 +10,17 @@ P2G, grid update, G2P.
 
 Prints one JSON line.  By default W = 5 and K = 195: the timed window is
 steps 6-200 of the configuration's 200-step dam break.  `value` times K
-steps as the library runs them: each
-step replays the captured substep hipGraph and returns after a stream
-synchronize, which brackets the region.  The particles are then put back to
-their state at the start of that window and the same K steps run again with
-profiling on -- the same kernels launched eagerly on the context stream, each
-stage's HIP event pair carried by its kernels' dispatches -- to give the
-per-kernel durations behind `roofline` (per-kernel events slow the step
-itself, so they are kept out of the window that gives `value`; replaying the
-same steps matters because P2G's cost grows as the scene evolves).  The
-launch shape is
+steps enqueued back to back through aemh_step_async (each replays the
+captured substep hipGraph) and one aemh_sync at the end, which brackets the
+region with the stream synchronized on both sides.  The particles are then
+put back to their state at the start of that window twice: the same K steps
+run once through aemh_step, which returns after a stream synchronize per step
+(`extra.synchronized_steps_particle_updates_per_s`), and once with profiling
+on -- the same kernels launched eagerly on the context stream, each stage's
+HIP event pair carried by its kernels' dispatches -- to give the per-kernel
+durations behind `roofline` (per-kernel events slow the step itself, so they
+are kept out of the window that gives `value`; replaying the same steps
+matters because P2G's cost grows as the scene evolves).  The launch shape is
 autotuned once before the warm-up.  `extra` adds the profiled window's own
 throughput, the 16M-particle sparse-grid workload, the deterministic Q32.32
 mode, the batched-scenes speedup, the mixed-precision workload, the
@@ -68,9 +69,20 @@ def log(msg):
 
 
 def timed_steps(eng, dt, k):
+    """K steps, each returning after the GPU finished it (aemh_step)."""
     t0 = time.perf_counter()
     for _ in range(k):
         eng.step(dt)
+    return time.perf_counter() - t0
+
+
+def timed_steps_async(eng, dt, k):
+    """K steps enqueued back to back (aemh_step_async), one synchronize at
+    the end: the GPU does not idle between steps for the host's round trip."""
+    t0 = time.perf_counter()
+    for _ in range(k):
+        eng.step_async(dt)
+    eng.sync()
     return time.perf_counter() - t0
 
 
@@ -105,7 +117,12 @@ def run_main(W, aemh, steps, warmup, accum=None):
     snap = snapshot(eng)
     for _ in range(pre):
         eng.step(wl.dt)
-    el = timed_steps(eng, wl.dt, steps)
+    el = timed_steps_async(eng, wl.dt, steps)
+    # the same steps again, each synchronized, then profiled
+    restore(eng, wl, snap, shape)
+    for _ in range(pre):
+        eng.step(wl.dt)
+    el_sync = timed_steps(eng, wl.dt, steps)
     restore(eng, wl, snap, shape)
     for _ in range(pre):
         eng.step(wl.dt)
@@ -116,7 +133,7 @@ def run_main(W, aemh, steps, warmup, accum=None):
     eng.profile(False)
     blocks, npart = eng.last_substep_info()
     eng.close()
-    return wl, el, el_prof, kern, blocks, npart, shape
+    return wl, el, el_sync, el_prof, kern, blocks, npart, shape
 
 
 def roofline_of(kern, name, npart, blocks, traffic=None, valu=None):
@@ -615,7 +632,7 @@ def main():
         return
 
     log("[bench] main workload: C2 1M snow particles, 128^3")
-    wl, el, el_prof, kern, blocks, npart, shape = run_main(W, aemh, args.steps, args.warmup)
+    wl, el, el_sync, el_prof, kern, blocks, npart, shape = run_main(W, aemh, args.steps, args.warmup)
     value = wl.n * args.steps / el
     dominant = max(("p2g", "g2p", "grid_update"), key=lambda k: kern[k])
     step_bytes = npart * BYTES["step"][0] + blocks * 64 * BYTES["step"][1]
@@ -641,14 +658,15 @@ def main():
         "step_hbm_frac": round(step_bytes / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
     }
-    extra = {"profiled_window_particle_updates_per_s": wl.n * args.steps / el_prof,
+    extra = {"synchronized_steps_particle_updates_per_s": wl.n * args.steps / el_sync,
+             "profiled_window_particle_updates_per_s": wl.n * args.steps / el_prof,
              "g2p_roofline": roofline_of(kern, "g2p", npart, blocks, TRAFFIC_C2, VALU_PER_WAVE_C2),
              "active_blocks": blocks}
     want = set() if args.no_extras else {e.strip() for e in args.extras.split(",") if e.strip()}
     if "q32" in want:
         log("[bench] deterministic Q32.32 mode")
-        wq, elq, elq_g, kq, _, _, _ = run_main(W, aemh, args.steps, args.warmup,
-                                                 accum=aemh.ACCUM_Q32_32)
+        wq, elq, _, _, _, _, _, _ = run_main(W, aemh, args.steps, args.warmup,
+                                             accum=aemh.ACCUM_Q32_32)
         extra["q32_particle_updates_per_s"] = wq.n * args.steps / elq
     if "c3" in want:
         log("[bench] C3 16M particles")

@@ -257,6 +257,17 @@ aemh_status aemh_mpm_step_parallel_with_grid(aemh_ctx* ctx, const uint32_t* idx,
                                              uint32_t n_idx, float total_dt);
 /* mpm_step_velocity_only_parallel (mpm_solver.rs:1337-1351). */
 aemh_status aemh_step_velocity_only(aemh_ctx* ctx, float total_dt);
+/* aemh_step without waiting for the GPU: no counterpart in the reference
+ * (whose step returns when done).  For callers that advance several steps
+ * between host reads (substep loops, benchmarks): the host enqueues the
+ * next step while the GPU runs the last.  Every later call on the context
+ * is ordered after it; downloads wait for it.  aemh_sync waits and returns
+ * the deterministic mode's range check (AEMH_ERR_OVERFLOW) of the steps
+ * since the last synchronous call; between synchronous calls the re-sort
+ * decision uses only the fixed cadence.  With a resident MpssBuffer (whose
+ * step ages lifetimes) it is aemh_step. */
+aemh_status aemh_step_async(aemh_ctx* ctx, float total_dt);
+aemh_status aemh_sync(aemh_ctx* ctx);
 
 /* Batched scenes: every scene advances in one launch per phase, the whole
  * substep captured in a hipGraph and replayed n_substeps times. */
