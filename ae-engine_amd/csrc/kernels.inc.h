@@ -184,6 +184,18 @@ __device__ __forceinline__ float pair_swap(float x) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
 }
 
+// Maximum of v over the wave, every lane of the wave active: DPP maxima
+// within each row of 16 lanes (quad permutations [1, 0, 3, 2] and [2, 3, 0,
+// 1], then the half-row and row mirrors), then the four rows' lane 0.
+__device__ __forceinline__ int wave_max_i32(int v) {
+  v = max(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false));
+  v = max(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false));
+  v = max(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false));
+  v = max(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false));
+  return max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+             max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
 // The 27-node scatter of a pair of lanes whose particles share a base cell
 // (the FP32 mode's 4 values per node: m and p_r + f_r dt, the same values as
 // p2g_nodes<true, true>).  At each node lane 2i keeps fields 0 and 2 of its
@@ -316,12 +328,18 @@ __device__ __forceinline__ long long rint_i64(double y) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(y + 6755399441055744.0);
   return (long long)(b - 0x4338000000000000ull);
 }
-// rint_i64(v * scale) for a power-of-two scale with |v scale| < 2^51, in
-// three VALU: the product is exact, so one fused multiply-add with the same
-// constant rounds exactly as the separate product and sum do.
-__device__ __forceinline__ long long rint_i64_scaled(float v, double scale, double magic) {
-  const double y = __builtin_fma((double)v, scale, magic);
-  return (long long)((unsigned long long)__double_as_longlong(y) - 0x4338000000000000ull);
+// rint(v * scale) for a power-of-two scale with |v scale| < 2^51, biased by
+// the bit pattern of 1.5 2^52, in two VALU: the product is exact, so one
+// fused multiply-add with that constant rounds exactly as the separate
+// product and sum do, and its bit pattern is 0x4338000000000000 + rint(v
+// scale).  The bias is a multiple of 2^51 (0x4338 = 8 * 0x867), so a sum of
+// such patterns modulo 2^64 keeps the sum of the rounded values in its low
+// 51 bits: unbias_sum51 recovers it whenever its magnitude is below 2^50.
+__device__ __forceinline__ unsigned long long rint_biased_scaled(float v, double scale, double magic) {
+  return (unsigned long long)__double_as_longlong(__builtin_fma((double)v, scale, magic));
+}
+__device__ __forceinline__ long long unbias_sum51(unsigned long long raw) {
+  return (long long)(raw << 13) >> 13;
 }
 
 #if AEMH_DET
@@ -358,8 +376,8 @@ __device__ __forceinline__ void tile_node_of(int t, int& lx, int& ly, int& lz) {
 // ds_add_f32 on gfx950.  The deterministic mode adds FixedPoint Q32.32 values;
 // the FP32 mode adds each value scaled by a per-workgroup power of two 2^s_g
 // (one per field group: mass, momentum, force) chosen so 256 contributions
-// cannot overflow, i.e. an exact block-floating-point sum with resolution
-// 2^-50 of the group's largest contribution.  Tiles are flushed block-
+// cannot leave 50 bits, i.e. an exact block-floating-point sum with
+// resolution 2^-42 of the group's largest contribution.  Tiles are flushed block-
 // coalesced with one HBM atomic per non-zero node value.  The LDS and HBM
 // scatter paths are separate code so the LDS one stays ds_add.
 template <int STORE, int TILES, bool PAIRS = false>
@@ -588,9 +606,7 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
   }
 #pragma unroll
   for (int g = 0; g < 2; g++) {
-    int e = finite_in ? ex[g] : -126;
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) e = max(e, __shfl_xor(e, d, 64));
+    const int e = wave_max_i32(finite_in ? ex[g] : -126);
     if (lane == 0) wexp[wave][g] = e;
   }
 #endif
@@ -604,7 +620,9 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
   for (int g = 0; g < 2; g++) {
     eg[g] = max(max(wexp[0][g], wexp[1][g]), max(wexp[2][g], wexp[3][g]));
     eg[g] = max(eg[g], -70);
-    scale[g] = (double)__int_as_float((127 + 50 - eg[g]) << 23);  // 2^(50 - e_g)
+    // 2^(42 - e_g): |contribution scale| < 2^41, so the sum of a tile
+    // node's at most 256 contributions stays below 2^49
+    scale[g] = (double)__int_as_float((127 + 42 - eg[g]) << 23);
   }
 #endif
 
@@ -654,9 +672,8 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
 #else
       auto put = [&](int f, int di, int dj, int dk, int, int, int, float v) {
         const int g = f == 0 ? 0 : 1;
-        const long long a = rint_i64_scaled(v, scale[g], magic);  // |v scale| < 2^50
         atomicAdd((unsigned long long*)&tile[tb + f * kTileNodes + (dk * kTileSide + dj) * kTileSide + di],
-                  (unsigned long long)a);
+                  rint_biased_scaled(v, scale[g], magic));
       };
 #endif
 #if AEMH_DET
@@ -673,10 +690,8 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
         const int f0 = (hi ? 1 : 0) * kTileNodes, f1 = (hi ? 3 : 2) * kTileNodes;
         auto put2 = [&](int di, int dj, int dk, float v0, float v1) {
           const int node = tb + (dk * kTileSide + dj) * kTileSide + di;
-          atomicAdd((unsigned long long*)&tile[node + f0],
-                    (unsigned long long)rint_i64_scaled(v0, sc0, magic));  // |v scale| < 2^50
-          atomicAdd((unsigned long long*)&tile[node + f1],
-                    (unsigned long long)rint_i64_scaled(v1, scale[1], magic));
+          atomicAdd((unsigned long long*)&tile[node + f0], rint_biased_scaled(v0, sc0, magic));
+          atomicAdd((unsigned long long*)&tile[node + f1], rint_biased_scaled(v1, scale[1], magic));
         };
         p2g_nodes_pair(n0, n1, n2, dx, o0, o1, o2, st, q, hi, put2);
       } else if (full) {
@@ -745,9 +760,10 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
       float v[kPartStride];
 #pragma unroll
       for (int f = 0; f < kPartStride; f++) {
-        const long long q = f < kTileFields ? tile[ti * kTileVals + f * kTileNodes + t] : 0;
+        const long long q =
+            f < kTileFields ? unbias_sum51((unsigned long long)tile[ti * kTileVals + f * kTileNodes + t]) : 0;
         const int g = f == 0 ? 0 : 1;
-        v[f] = q == 0 ? 0.0f : (float)((double)q * (double)__int_as_float((127 - 50 + eg[g]) << 23));
+        v[f] = q == 0 ? 0.0f : (float)((double)q * (double)__int_as_float((127 - 42 + eg[g]) << 23));
       }
       *reinterpret_cast<float4*>(G.partf + rec) = make_float4(v[0], v[1], v[2], v[3]);
 #endif
@@ -768,13 +784,17 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
       const uint32_t node = (b << 6) | (uint32_t)(((gz & 3) << 4) | ((gy & 3) << 2) | (gx & 3));
 #pragma unroll
       for (int f = 0; f < kTileFields; f++) {
+#if AEMH_DET
         const long long v = tile[ti * kTileVals + f * kTileNodes + li];
+#else
+        const long long v = unbias_sum51((unsigned long long)tile[ti * kTileVals + f * kTileNodes + li]);
+#endif
         if (v == 0) continue;
 #if AEMH_DET
         atomicAdd((unsigned long long*)(G.accq + (size_t)f * nodes + node), (unsigned long long)v);
 #else
         const int g = f == 0 ? 0 : 1;
-        const float fv = (float)((double)v * (double)__int_as_float((127 - 50 + eg[g]) << 23));
+        const float fv = (float)((double)v * (double)__int_as_float((127 - 42 + eg[g]) << 23));
         unsafeAtomicAdd(G.accf + (size_t)f * nodes + node, fv);
 #endif
       }
