@@ -152,9 +152,14 @@ struct aemh_ctx {
   LaunchShape shape;
   bool tuned = false;
   float tune_ms[3] = {0.0f, 0.0f, 0.0f};  // best P2G, grid update, G2P time
-  // graphs: one per (ping-pong parity, re-sort step), built on first use
-  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
+  // graphs: one per (ping-pong parity, re-sort step, count kernel or not),
+  // built on first use
+  hipGraphExec_t gexec[8] = {};
   bool graph_valid = false;
+  // The last substep's G2P counted the sort of work[cur] as it stands (keys,
+  // ranks, block counts and wave runs), so the next substep skips the count
+  // kernel.  Cleared, with the counts, by any change made between substeps.
+  bool counted = false;
   // in-place substeps between re-sorting ones
   uint32_t since_sort = 0;
   bool need_sort = true;
@@ -267,6 +272,15 @@ size_t store_bytes(uint32_t n, int storage) {
   return (size_t)((char*)pb.id - base) + (((size_t)n * 4 + 255) & ~(size_t)255) + kPlanePad;
 }
 
+// Both copies of the working set fit in the Infinity Cache.  Such sets
+// re-sort every kResortResident-th substep, and their G2P counts the next
+// substep's sort: on the 1M-particle snow workload the count kernel's 9.8 us
+// go for 5.4 us more G2P.  On larger sets G2P grows by as much as the count
+// kernel takes (16M particles: 93 us against 97 us), and they keep it.
+bool cache_resident(const aemh_ctx* c) {
+  return 2 * store_bytes(c->n_work, c->work[0].storage) <= kInfinityCacheBytes;
+}
+
 aemh_status alloc_store(aemh_ctx* c, ParticleStore& s, uint32_t n, int storage) {
   if (s.slab && s.cap >= n && s.storage == storage) return AEMH_OK;
   if (s.slab) {
@@ -292,13 +306,27 @@ aemh_status ensure_staging(aemh_ctx* c, size_t bytes) {
   return AEMH_OK;
 }
 
+// The block counts and wave runs G2P left for the next substep are zeroed
+// (stream-ordered after it) and that substep runs the count kernel.
+void drop_counts(aemh_ctx* c) {
+  if (!c->counted) return;
+  c->counted = false;
+  hipError_t e = hipMemsetAsync(c->sort.cnt, 0, (size_t)c->sort_nb_cap * 4, c->stream);
+  if (e == hipSuccess)
+    e = hipMemsetAsync(c->sort.runs, 0, (size_t)kRunLanes * kRunStride * 4, c->stream);
+  if (e != hipSuccess) c->err = std::string("clearing the sort counts: ") + hipGetErrorName(e);
+}
+
+// Every change that rebuilds the graphs (buffers, launch shape, working set,
+// configuration, materials) also drops the counts G2P made.
 void invalidate_graphs(aemh_ctx* c) {
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < 8; i++)
     if (c->gexec[i]) {
       hipGraphExecDestroy(c->gexec[i]);
       c->gexec[i] = nullptr;
     }
   c->graph_valid = false;
+  drop_counts(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -482,6 +510,7 @@ aemh_status set_scene_table(aemh_ctx* c, const std::vector<aemh_scene>& sc) {
                     memcmp(c->scd_h.data(), d.data(), d.size() * sizeof(SceneDev)) == 0;
   c->scd_h = d;
   if (!same) {
+    drop_counts(c);  // keys are functions of the scene table
     HIPTRY(c, hipMemcpyAsync(c->scd_d, d.data(), d.size() * sizeof(SceneDev), hipMemcpyHostToDevice,
                              c->stream));
     c->scd_valid = true;
@@ -728,15 +757,19 @@ aemh_status ensure_serial(aemh_ctx* c) {
 // Launch one substep on the context stream (eager or under capture).
 // One substep on work[cur].  A re-sort step writes G2P's result to
 // work[cur ^ 1] in block order; any other step updates work[cur] in place.
-void enqueue_substep(aemh_ctx* c, int cur, bool resort) {
+// G2P of a cache-resident working set counts the next substep's sort; the
+// count kernel runs when the previous substep's G2P did not (count_kernel).
+void enqueue_substep(aemh_ctx* c, int cur, bool resort, bool count_kernel) {
   const PBuf a = c->work[cur].pb, b = resort ? c->work[cur ^ 1].pb : a;
   const int inplace = resort ? 0 : 1;
   const uint32_t n = c->n_work;
   const int ns = (int)c->scd_h.size();
   const int storage = c->work[0].storage;
   const int q32 = c->cfg.accum == AEMH_ACCUM_Q32_32;
+  const int fuse = cache_resident(c) ? 1 : 0;
   hipStream_t st = c->stream;
-  timed_k(c, "count", [&] { launch_count(st, a, c->scd_d, ns, n, c->sort, c->nb_total); });
+  if (count_kernel)
+    timed_k(c, "count", [&] { launch_count(st, a, c->scd_d, ns, n, c->sort, c->nb_total); });
   timed_k(c, "scan", [&] { launch_scan(st, c->scd_d, ns, c->sort, c->nb_total); });
   timed_k(c, "scatter", [&] { launch_scatter(st, n, c->nb_total, c->sort, c->grid, acc_layout(c)); });
   if (c->cfg.accum == AEMH_ACCUM_F32_SERIAL) {
@@ -749,7 +782,7 @@ void enqueue_substep(aemh_ctx* c, int cur, bool resort) {
     });
     timed_k(c, "grid_update", [&] { fast::launch_grid_update_mpf(st, ls, c->scd_d, ns, c->sort, c->grid, n, c->nb_total); });
     timed_k(c, "g2p", [&] {
-      det::launch_g2p(st, ls, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, inplace);
+      det::launch_g2p(st, ls, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, inplace, fuse);
     });
   } else if (q32) {
     timed_k(c, "p2g", [&] {
@@ -757,7 +790,7 @@ void enqueue_substep(aemh_ctx* c, int cur, bool resort) {
     });
     timed_k(c, "grid_update", [&] { det::launch_grid_update(st, c->shape, c->scd_d, ns, c->sort, c->grid, n, c->nb_total); });
     timed_k(c, "g2p", [&] {
-      det::launch_g2p(st, c->shape, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, inplace);
+      det::launch_g2p(st, c->shape, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, inplace, fuse);
     });
   } else {
     timed_k(c, "p2g", [&] {
@@ -765,18 +798,22 @@ void enqueue_substep(aemh_ctx* c, int cur, bool resort) {
     });
     timed_k(c, "grid_update", [&] { fast::launch_grid_update(st, c->shape, c->scd_d, ns, c->sort, c->grid, n, c->nb_total); });
     timed_k(c, "g2p", [&] {
-      fast::launch_g2p(st, c->shape, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, inplace);
+      fast::launch_g2p(st, c->shape, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, inplace, fuse);
     });
   }
 }
 
-aemh_status build_graph(aemh_ctx* c, int cur, bool resort) {
+int graph_index(int cur, bool resort, bool count_kernel) {
+  return cur * 4 + (resort ? 2 : 0) + (count_kernel ? 1 : 0);
+}
+
+aemh_status build_graph(aemh_ctx* c, int cur, bool resort, bool count_kernel) {
   hipGraph_t g = nullptr;
   HIPTRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  enqueue_substep(c, cur, resort);
+  enqueue_substep(c, cur, resort, count_kernel);
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (e != hipSuccess) return fail(c, AEMH_ERR_HIP, "graph capture: %s", hipGetErrorName(e));
-  e = hipGraphInstantiate(&c->gexec[cur * 2 + (resort ? 1 : 0)], g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&c->gexec[graph_index(cur, resort, count_kernel)], g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if (e != hipSuccess) return fail(c, AEMH_ERR_HIP, "graph instantiate: %s", hipGetErrorName(e));
   c->graph_valid = true;
@@ -837,22 +874,24 @@ aemh_status run_substeps(aemh_ctx* c, uint32_t k, bool allow_graph = true, bool 
     // Re-sort on the first substep of a new working set, when the block
     // order has decayed, and at least every kResortMax substeps.
     const uint32_t every = c->cfg.resort_interval;
-    const bool resident = 2 * store_bytes(c->n_work, c->work[0].storage) <= kInfinityCacheBytes;
+    const bool resident = cache_resident(c);
     const bool resort = every ? (c->need_sort || c->since_sort + 1 >= every)
                               : (c->need_sort ||
                                  c->since_sort + 1 >= (resident ? kResortResident : kResortMax));
     counted_sorted = c->fresh_sort;
+    const bool count_kernel = !c->counted;
     if (graph) {
-      hipGraphExec_t& ge = c->gexec[c->cur * 2 + (resort ? 1 : 0)];
+      hipGraphExec_t& ge = c->gexec[graph_index(c->cur, resort, count_kernel)];
       if (!ge) {
-        aemh_status s = build_graph(c, c->cur, resort);
+        aemh_status s = build_graph(c, c->cur, resort, count_kernel);
         if (s != AEMH_OK) return s;
       }
       HIPTRY(c, hipGraphLaunch(ge, c->stream));
     } else {
-      enqueue_substep(c, c->cur, resort);
+      enqueue_substep(c, c->cur, resort, count_kernel);
       CHECK_LAUNCH(c);
     }
+    c->counted = resident;  // G2P counted the next substep's sort
     if (resort) {
       c->cur ^= 1;
       c->since_sort = 0;
@@ -1162,6 +1201,7 @@ aemh_status aemh_destroy(aemh_ctx* c) {
   if (!c) return AEMH_OK;
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
+  c->counted = false;  // nothing to clear: the buffers go
   invalidate_graphs(c);
   for (auto& pr : c->pending) {
     hipEventDestroy(pr.second.first);
@@ -1492,6 +1532,7 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   const int q32 = c->cfg.accum == AEMH_ACCUM_Q32_32;
   hipStream_t st = c->stream;
   LaunchShape& ls = c->shape;
+  drop_counts(c);  // the timing runs count and sort afresh
   launch_count(st, a, c->scd_d, ns, n, c->sort, c->nb_total);
   launch_scan(st, c->scd_d, ns, c->sort, c->nb_total);
   launch_scatter(st, n, c->nb_total, c->sort, c->grid, acc_layout(c));
@@ -1554,9 +1595,9 @@ aemh_status aemh_autotune(aemh_ctx* c) {
     ls.g2p_wg = wg;
     s = time_launches(c, kReps, &ms, [&] {
       if (q32)
-        det::launch_g2p(st, ls, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, 1);
+        det::launch_g2p(st, ls, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, 1, 0);
       else
-        fast::launch_g2p(st, ls, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, 1);
+        fast::launch_g2p(st, ls, a, b, c->scd_d, ns, c->mats_d, (int)c->nmat, n, c->sort, c->grid, storage, 1, 0);
     });
     if (s != AEMH_OK) return s;
     if (ms < best) best = ms, pick = wg;

@@ -71,6 +71,11 @@ __device__ __forceinline__ uint32_t block_key(const SceneDev& s, const Stencil& 
   int bz = min(st.g[2], s.n[2] - 1) >> 2;
   return s.block_off + (uint32_t)((bz * s.nb[1] + by) * s.nb[0] + bx);
 }
+// Runs of equal keys seen by one wave (how far the working set is from block
+// order), spread over kRunLanes counters on separate 64 B lines.
+__device__ __forceinline__ void count_runs(SortDev& s, uint32_t wave_id, uint32_t runs) {
+  atomicAdd(&s.runs[(wave_id % (uint32_t)kRunLanes) * kRunStride], runs);
+}
 // Node address in the block-major grid (4^3 nodes per block, x fastest).
 __device__ __forceinline__ uint32_t node_addr(const SceneDev& s, int gx, int gy, int gz) {
   uint32_t b = s.block_off + (uint32_t)(((gz >> 2) * s.nb[1] + (gy >> 2)) * s.nb[0] + (gx >> 2));

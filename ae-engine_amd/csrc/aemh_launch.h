@@ -124,7 +124,7 @@ void launch_gather_pos(hipStream_t st, PBuf P, const uint32_t* list, uint32_t n,
                           int ns, SortDev s, GridDev g, uint32_t n, uint32_t nb_total);     \
   void launch_g2p(hipStream_t st, const LaunchShape& ls, PBuf cur, PBuf nxt,                \
                   const SceneDev* sc, int ns, const MatDev* mats, int nmat, uint32_t n,     \
-                  SortDev s, GridDev g, int storage, int inplace);                          \
+                  SortDev s, GridDev g, int storage, int inplace, int count);               \
   }
 AEMH_DECLARE_MODE(fast)
 AEMH_DECLARE_MODE(det)

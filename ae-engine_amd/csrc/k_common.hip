@@ -8,12 +8,6 @@
 namespace aemh {
 namespace {
 
-// Runs of equal keys seen by one wave (how far the working set is from block
-// order), spread over kRunLanes counters on separate 64 B lines.
-__device__ __forceinline__ void count_runs(SortDev& s, uint32_t wave_id, uint32_t runs) {
-  atomicAdd(&s.runs[(wave_id % (uint32_t)kRunLanes) * kRunStride], runs);
-}
-
 // Key + rank: every working slot computes the global block of its clamped
 // base cell; runs of equal keys inside a wave take one atomic on the block
 // counter, so nearly sorted input costs about one atomic per wave.

@@ -397,7 +397,9 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
   __shared__ int tdim[TILES][5];  // scene grid n[0..2], nb[0..1] of the tile
   __shared__ uint32_t tboff[TILES];
   __shared__ uint32_t wcnt[4];
-  __shared__ int wexp[4][2];
+#if !AEMH_DET
+  __shared__ int wexp[4][2];  // per wave: block exponent of each field group
+#endif
 
   const int t = threadIdx.x;
   const int lane = __lane_id(), wave = t >> 6;
@@ -952,14 +954,52 @@ __device__ __forceinline__ void node_axes(uint32_t boff, int nb0, int nb1, int x
   az = ((boff + (uint32_t)((z >> 2) * nb1 * nb0)) << 6) | ((uint32_t)(z & 3) << 4);
 }
 
+// The next substep's sort count, made by G2P where each new position is
+// stored (slot j of the output): the count kernel's block key of that
+// position and the particle's rank among the block's particles, one atomic
+// per run of equal keys in a wave, as the count kernel does for slot j at the
+// start of the next substep.  A wave whose lanes are split between a
+// velocity-only scene and a full one counts lane by lane.  Ranks within a
+// block may come in any order (so do the count kernel's).
+__device__ __forceinline__ void count_next(SortDev& s, const SceneDev& S, float x0, float x1, float x2,
+                                           uint32_t j, uint32_t n) {
+  const uint32_t key = block_key(S, make_stencil(S, x0, x1, x2));
+  s.key[j] = key;
+  const int lane = __lane_id();
+  const uint32_t first = j - (uint32_t)lane;  // slot of lane 0
+  const uint32_t nv = min(64u, n - first);    // slots of this wave below n
+  const unsigned long long all = nv == 64u ? ~0ull : ((1ull << nv) - 1ull);
+  const unsigned long long act = __ballot(1);
+  if (act != all) {
+    s.rank[j] = atomicAdd(&s.cnt[key], 1u);
+    if (lane == __ffsll((long long)act) - 1) count_runs(s, first >> 6, (uint32_t)__popcll(act));
+    return;
+  }
+  const uint32_t prev = __shfl_up(key, 1, 64);
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  uint32_t old = 0;
+  if (head) {
+    const unsigned long long above = heads & ~le;
+    const int next = above ? (__ffsll((long long)above) - 1) : (int)nv;
+    old = atomicAdd(&s.cnt[key], (uint32_t)(next - lane));
+  }
+  const int h = 63 - __clzll((long long)(heads & le));
+  s.rank[j] = __shfl(old, h, 64) + (uint32_t)(lane - h);
+  if (lane == 0) count_runs(s, first >> 6, (uint32_t)__popcll(heads));
+}
+
 // G2P.  Re-sort step (inplace = 0): sorted position j advances working slot
 // p = perm[j] of `cur` and writes slot j of `nxt` with the particle's mass,
 // material and id, so the arrays come out ordered by base block.  In-place
 // step (inplace = 1): thread j advances slot j and writes it back to slot j
 // of `nxt` (`cur` itself, or a scratch copy for timing), leaving the fields
 // G2P does not change; particles cross blocks rarely, so between re-sorts a
-// workgroup's slots still fall in a few blocks.
-template <int STORE>
+// workgroup's slots still fall in a few blocks.  With COUNT, G2P also counts
+// the next substep's sort (count_next), and that substep skips the count
+// kernel; the instantiation without it carries none of that code.
+template <int STORE, bool COUNT>
 __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const SceneDev* __restrict__ sc,
                                              int ns, const MatDev* __restrict__ mats, int nmat,
                                              uint32_t n, SortDev s, GridDev G, int inplace) {
@@ -1182,6 +1222,7 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
     at4(nxt.x[0], os) = x0;
     at4(nxt.x[1], os) = x1;
     at4(nxt.x[2], os) = x2;
+    if constexpr (COUNT) count_next(s, S, x0, x1, x2, j, n);
     at4(nxt.v[0], os) = nv0 * S.damping;
     at4(nxt.v[1], os) = nv1 * S.damping;
     at4(nxt.v[2], os) = nv2 * S.damping;
@@ -1435,6 +1476,7 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
   at4(nxt.x[0], os) = np[0];
   at4(nxt.x[1], os) = np[1];
   at4(nxt.x[2], os) = np[2];
+  if constexpr (COUNT) count_next(s, S, np[0], np[1], np[2], j, n);
   at4(nxt.v[0], os) = nv[0] * S.damping;
   at4(nxt.v[1], os) = nv[1] * S.damping;
   at4(nxt.v[2], os) = nv[2] * S.damping;
@@ -1628,17 +1670,26 @@ void launch_grid_update_mpf(hipStream_t st, const LaunchShape& ls, const SceneDe
 
 void launch_g2p(hipStream_t st, const LaunchShape& ls, PBuf cur, PBuf nxt, const SceneDev* sc,
                 int ns, const MatDev* mats, int nmat, uint32_t n, SortDev s, GridDev g,
-                int storage, int inplace) {
+                int storage, int inplace, int count) {
   if (!n) return;
   const unsigned wg = (unsigned)ls.g2p_wg;
   const unsigned grid = (n + wg - 1u) / wg;
   const size_t lds = (size_t)g2p_tiles_for((int)wg) * kTileNodes * sizeof(float4);
-  if (storage == 1)
-    launch_k(k_g2p<1>, grid, wg, lds, st, cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
-  else if (storage == 2)
-    launch_k(k_g2p<2>, grid, wg, lds, st, cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
-  else
-    launch_k(k_g2p<0>, grid, wg, lds, st, cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
+  if (count) {
+    if (storage == 1)
+      launch_k(k_g2p<1, true>, grid, wg, lds, st, cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
+    else if (storage == 2)
+      launch_k(k_g2p<2, true>, grid, wg, lds, st, cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
+    else
+      launch_k(k_g2p<0, true>, grid, wg, lds, st, cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
+  } else {
+    if (storage == 1)
+      launch_k(k_g2p<1, false>, grid, wg, lds, st, cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
+    else if (storage == 2)
+      launch_k(k_g2p<2, false>, grid, wg, lds, st, cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
+    else
+      launch_k(k_g2p<0, false>, grid, wg, lds, st, cur, nxt, sc, ns, mats, nmat, n, s, g, inplace);
+  }
 }
 
 }  // namespace AEMH_NS

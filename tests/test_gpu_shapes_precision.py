@@ -92,7 +92,8 @@ def test_autotune_does_not_advance_state():
 
 def test_profiled_steps_are_exact_and_timed():
     """Profiled steps launch eagerly with events around each section: results
-    stay bit-exact with the graph replays and every section gets a time."""
+    stay bit-exact with the graph replays and every section gets a time.  The
+    count kernel does not run: each step's G2P counted the next step's sort."""
     wl = W.c1_elastic_cube()
     wl.accum = aemh.ACCUM_Q32_32
     st = oracle_run(wl, 6)
@@ -103,12 +104,14 @@ def test_profiled_steps_are_exact_and_timed():
     eng.profile_reset()
     for _ in range(4):
         eng.step(wl.dt)
-    times = {k: eng.profile_get(k) for k in ("count", "scan", "p2g", "grid_update", "g2p")}
+    times = {k: eng.profile_get(k) for k in ("scan", "scatter", "p2g", "grid_update", "g2p")}
+    counts = eng.profile_get("count")[1]
     eng.profile(False)
     eng.step(wl.dt)
     assert_bitexact(eng.download(), st)
     for k, (ms, n) in times.items():
         assert n == 4 and 0.0 < ms < 50.0, (k, ms, n)
+    assert counts == 0
     eng.close()
 
 
