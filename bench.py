@@ -47,21 +47,26 @@ HBM_PEAK_GBS = 8000.0
 #   whole step 206 B + 8 B (J_p read+write)                         = 214 B
 # per active grid node: P2G writes 28 B (m, p, f); update reads 28, writes
 #   16 (v, m); G2P reads 16 -> step 88 B; P2G 28 B; G2P 16 B.
-BYTES = {"p2g": (110, 28), "g2p": (156, 16), "grid_update": (0, 44), "step": (214, 88)}
+# A G2P that also counts the next substep's sort (cache-resident working
+# sets; the count kernel then does not run) writes each particle's block key
+# and rank, 8 B more.
+BYTES = {"p2g": (110, 28), "g2p": (156, 16), "g2p_count": (164, 16), "grid_update": (0, 44),
+         "step": (214, 88)}
 KERNELS = ["count", "scan", "scatter", "p2g", "grid_update", "g2p"]
 # HBM bytes per dispatch from rocprofv3 FETCH_SIZE (x 2.000) and WRITE_SIZE
 # (x 1.000), calibrated against a 256 MiB copy, as recorded in
 # profiles/p2g_g2p_sq_counters.md (1M-particle snow workload, launch shape
-# 4,1,256,2048; 16M-particle workload, early steps, 4,0,256,8192).
-TRAFFIC_C2 = {"p2g": 191.5e6, "g2p": 224.8e6, "grid_update": 90.7e6}
-TRAFFIC_C3 = {"p2g": 2637e6, "g2p": 3068e6}
+# 4,1,256,8192; 16M-particle workload, early steps, 4,0,256,8192).
+TRAFFIC_C2 = {"p2g": 189.9e6, "g2p": 220.4e6, "grid_update": 90.8e6}
+TRAFFIC_C3 = {"p2g": 2578e6, "g2p": 3068e6}
 # VALU instructions per wave (one wave = 64 particles) from rocprofv3
-# SQ_INSTS_VALU / SQ_WAVES, recorded in profiles/p2g_g2p_sq_counters.md.  The
-# issue ceiling without packed FP32: one wave64 VALU instruction per 4 cycles
-# per SIMD, 4 SIMDs x 256 CUs at 2.4 GHz = 614.4e9 wave-instructions/s.
+# SQ_INSTS_VALU / SQ_WAVES, recorded in profiles/p2g_g2p_sq_counters.md; P2G
+# without and with the pair reduction.  The issue ceiling without packed
+# FP32: one wave64 VALU instruction per 4 cycles per SIMD, 4 SIMDs x 256 CUs
+# at 2.4 GHz = 614.4e9 wave-instructions/s.
 VALU_PEAK_WINST = 256 * 4 * 2.4e9 / 4
-VALU_PER_WAVE_C2 = {"p2g": 1806, "g2p": 1421}
-VALU_PER_WAVE_C3 = {"p2g": 1831, "g2p": 1295}
+VALU_PER_WAVE_C2 = {"p2g": (1581, 1737), "g2p": 1521}
+VALU_PER_WAVE_C3 = {"p2g": (1594, 1790), "g2p": 1294}
 
 
 def log(msg):
@@ -136,12 +141,15 @@ def run_main(W, aemh, steps, warmup, accum=None):
     return wl, el, el_sync, el_prof, kern, blocks, npart, shape
 
 
-def roofline_of(kern, name, npart, blocks, traffic=None, valu=None):
+def roofline_of(kern, name, npart, blocks, traffic=None, valu=None, pairs=0):
     """HBM roofline of one kernel: algorithmic bytes per launch over its average
     HIP-event duration.  With `valu` (VALU instructions per wave from the
-    counter profile), also the VALU issue rate against its ceiling: P2G is
-    limited by VALU issue and LDS atomics long before HBM."""
-    per_p, per_n = BYTES[name]
+    counter profile; P2G's by its pair reduction flag), also the VALU issue
+    rate against its ceiling: P2G is limited by VALU issue and LDS atomics
+    long before HBM.  A G2P that counted the next substep's sort (no count
+    kernel in the window) is charged its key and rank writes."""
+    counted = name == "g2p" and kern.get("count", 0.0) == 0.0
+    per_p, per_n = BYTES["g2p_count" if counted else name]
     nbytes = npart * per_p + blocks * 64 * per_n
     ms = kern[name]
     achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
@@ -150,8 +158,9 @@ def roofline_of(kern, name, npart, blocks, traffic=None, valu=None):
            "traffic": None if traffic is None else int(traffic.get(name, 0)) or None,
            "kernel": name, "algorithmic_bytes": int(nbytes), "avg_ms": round(ms, 4)}
     if valu and name in valu and ms > 0:
-        rate = (npart + 63) // 64 * valu[name] / (ms * 1e-3)
-        out["valu_issue"] = {"insts_per_wave": valu[name], "achieved": round(rate / 1e9, 1),
+        per_wave = valu[name][1 if pairs else 0] if isinstance(valu[name], tuple) else valu[name]
+        rate = (npart + 63) // 64 * per_wave / (ms * 1e-3)
+        out["valu_issue"] = {"insts_per_wave": per_wave, "achieved": round(rate / 1e9, 1),
                              "peak": VALU_PEAK_WINST / 1e9, "unit": "G wave-instructions/s",
                              "frac": round(rate / VALU_PEAK_WINST, 4)}
     return out
@@ -270,7 +279,8 @@ def extra_c3(W, aemh, steps=96):
         "active_block_fraction": blocks / total_blocks,
         "step_hbm_frac": round(step_bytes / (el / steps) / 1e9 / HBM_PEAK_GBS, 4),
         "g2p": roofline_of(kern, "g2p", npart, blocks, TRAFFIC_C3, VALU_PER_WAVE_C3),
-        "p2g": roofline_of(kern, "p2g", npart, blocks, TRAFFIC_C3, VALU_PER_WAVE_C3),
+        "p2g": roofline_of(kern, "p2g", npart, blocks, TRAFFIC_C3, VALU_PER_WAVE_C3,
+                           shape["p2g_pairs"]),
         "ms_by_window": windows,
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
         "launch_shape": shape,
@@ -654,7 +664,8 @@ def main():
                    "particles": wl.n, "grid": [128, 128, 128], "accum": "f32", "substeps": 1,
                    "launch_shape": {k: shape[k] for k in ("p2g_tiles", "p2g_partials", "p2g_pairs",
                                                             "g2p_wg", "upd_wgs")}},
-        "roofline": roofline_of(kern, dominant, npart, blocks, TRAFFIC_C2, VALU_PER_WAVE_C2),
+        "roofline": roofline_of(kern, dominant, npart, blocks, TRAFFIC_C2, VALU_PER_WAVE_C2,
+                                shape["p2g_pairs"]),
         "step_hbm_frac": round(step_bytes / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
     }
