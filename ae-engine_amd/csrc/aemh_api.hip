@@ -1542,16 +1542,19 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   int pick = ls.p2g_tiles, pick_part = ls.p2g_partials, pick_pairs = ls.p2g_pairs;
   // Tile budget and flush mode together: the flush mode moves work between
   // P2G and the grid update, so both are timed.  The FP32 mode also times
-  // the pair reduction.  Each tile fewer than the most is charged 3 %: the
-  // state being timed is often freshly sorted, where a workgroup's particles
-  // span few blocks, and a smaller budget that ties there sends particles to
-  // the slow global path once the scene spreads (1M snow: 3 tiles tied at
-  // the tuning step, then took 106 us per P2G against 82 us with 4).
+  // the pair reduction.  The state being timed is often freshly sorted,
+  // where a workgroup's particles span few blocks: a smaller tile budget
+  // that wins there sends particles to the slow global path once the scene
+  // spreads (1M snow: 3 tiles won at the tuning step, then took 106 us per
+  // P2G against 82 us with 4).  FP32 tiles (4 sums, 6.9 KB) never limit
+  // occupancy, so that mode keeps the full budget; the deterministic
+  // mode's (8 sums, 13.8 KB) do, and there each tile fewer than the most is
+  // charged 3 %.
   s = ensure_partials(c);
   if (s != AEMH_OK) return s;
   for (int pairs = 0; pairs <= (q32 ? 0 : 1); pairs++)
   for (int part = 0; part <= 1; part++)
-    for (int tiles = 2; tiles <= 4; tiles++) {
+    for (int tiles = q32 ? 2 : kMaxTileBudget; tiles <= kMaxTileBudget; tiles++) {
       ls.p2g_tiles = tiles;
       ls.p2g_partials = part;
       ls.p2g_pairs = pairs;

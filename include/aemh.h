@@ -274,8 +274,9 @@ aemh_status aemh_sync(aemh_ctx* ctx);
 aemh_status aemh_set_scenes(aemh_ctx* ctx, const aemh_scene* scenes, uint32_t n);
 aemh_status aemh_batch_step(aemh_ctx* ctx, uint32_t n_substeps);
 /* Times candidate launch shapes on the current particles and scenes and
- * keeps the fastest of each (a P2G tile budget below 4 must beat 4 by 3 %
- * per tile): P2G LDS tiles per workgroup (2/3/4) together
+ * keeps the fastest of each: P2G LDS tiles per workgroup (deterministic
+ * mode: 2/3/4, a budget below 4 must beat 4 by 3 % per tile; FP32 mode: 4)
+ * together
  * with the tile flush (one HBM atomic per node value, or partial sums stored
  * per tile and added by the grid update) and, in the FP32 mode, the pair
  * reduction (lanes holding two particles of one base cell add their node
