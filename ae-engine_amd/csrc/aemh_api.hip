@@ -436,7 +436,7 @@ aemh_status ensure_grid_sort(aemh_ctx* c) {
     invalidate_graphs(c);
   }
   if (nb > c->sort_nb_cap || !c->sort.cnt) {
-    for (void* p : {(void*)c->sort.cnt, (void*)c->sort.off, (void*)c->sort.tsum,
+    for (void* p : {(void*)c->sort.cnt, (void*)c->sort.off, (void*)c->sort.tsum, (void*)c->sort.traw,
                     (void*)c->sort.flag, (void*)c->sort.active, (void*)c->sort.bslot})
       if (p) hipFree(p);
     const uint32_t tiles = (nb + kScanTile - 1) / kScanTile;
@@ -444,6 +444,7 @@ aemh_status ensure_grid_sort(aemh_ctx* c) {
     HIPTRY(c, hipMemset(c->sort.cnt, 0, (size_t)nb * 4));
     HIPTRY(c, hipMalloc(&c->sort.off, (size_t)nb * 4));
     HIPTRY(c, hipMalloc(&c->sort.tsum, (size_t)(tiles + 1) * 4));
+    HIPTRY(c, hipMalloc(&c->sort.traw, (size_t)(tiles + 1) * 4));
     HIPTRY(c, hipMalloc(&c->sort.flag, (size_t)nb * 4));
     HIPTRY(c, hipMemset(c->sort.flag, 0, (size_t)nb * 4));
     HIPTRY(c, hipMalloc(&c->sort.active, (size_t)nb * 4));
@@ -1211,7 +1212,7 @@ aemh_status aemh_destroy(aemh_ctx* c) {
   for (void* p : {(void*)c->mats_d, c->canon.slab, c->work[0].slab, c->work[1].slab,
                   (void*)c->idx_d, (void*)c->staging, (void*)c->scd_d, (void*)c->grid.accf,
                   (void*)c->grid.accq, (void*)c->grid.gv, (void*)c->sort.key, (void*)c->sort.rank,
-                  (void*)c->sort.perm, (void*)c->sort.cnt, (void*)c->sort.off, (void*)c->sort.tsum,
+                  (void*)c->sort.perm, (void*)c->sort.cnt, (void*)c->sort.off, (void*)c->sort.tsum, (void*)c->sort.traw,
                   (void*)c->sort.flag, (void*)c->sort.active, (void*)c->sort.counters,
                   (void*)c->sort.runs,
                   (void*)c->sort.bslot, (void*)c->grid.partf, (void*)c->grid.partq,

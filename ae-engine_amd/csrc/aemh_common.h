@@ -115,7 +115,8 @@ struct SortDev {
   uint32_t* perm;       // [N] sorted position -> working slot
   uint32_t* cnt;        // [NB] particles per base block (zeroed by the scan)
   uint32_t* off;        // [NB] tile-local exclusive prefix
-  uint32_t* tsum;       // [tiles] tile sums, scanned in place
+  uint32_t* traw;       // [tiles] tile sums of the count scan
+  uint32_t* tsum;       // [tiles] their exclusive scan
   uint32_t* flag;       // [NB] node block active this substep
   uint32_t* active;     // [NB] list of active node blocks
   uint32_t* counters;   // [8], see kCnt*

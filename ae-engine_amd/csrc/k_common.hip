@@ -231,10 +231,31 @@ __device__ __forceinline__ void activate_halo(const SceneDev* __restrict__ sc, i
       }
 }
 
+// The substep's bookkeeping, before the scatter counts the active blocks:
+// the per-substep counters (not the sticky overflow flag) are zeroed, and the
+// count's wave runs go to the host-mapped word (their counters cleared for
+// the next count).  One workgroup.
+__device__ __forceinline__ void start_substep(SortDev& s) {
+  if (threadIdx.x < kCntOverflow) s.counters[threadIdx.x] = 0u;
+  if (threadIdx.x < 64) {
+    uint32_t r = s.runs[threadIdx.x * kRunStride];
+    s.runs[threadIdx.x * kRunStride] = 0u;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) r += __shfl_down(r, d, 64);
+    if (threadIdx.x == 0 && s.runs_host) *(volatile uint32_t*)s.runs_host = r;
+  }
+}
+
+// At most this many scan tiles (65,536 blocks): the scatter's waves scan the
+// tile sums themselves, and the count scan's first workgroup does the
+// substep's bookkeeping, so no tile-sum kernel runs.
+constexpr uint32_t kScatterScanTiles = 64;
+
 // Exclusive scan of the per-block counts, tile of 1024 per workgroup; zeroes
 // the counts for the next substep and activates the halos of non-empty blocks.
 __global__ __launch_bounds__(256) void k_scan_tiles(const SceneDev* __restrict__ sc, int ns,
-                                                    SortDev s, uint32_t nb) {
+                                                    SortDev s, uint32_t nb, int book) {
+  if (book && blockIdx.x == 0) start_substep(s);
   __shared__ uint32_t wtot[4];
   const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 4u;
   uint32_t v[4];
@@ -255,30 +276,20 @@ __global__ __launch_bounds__(256) void k_scan_tiles(const SceneDev* __restrict__
     if (base + k < nb) s.off[base + k] = run;
     run += v[k];
   }
-  if (threadIdx.x == 255) s.tsum[blockIdx.x] = wpre + incl;
+  if (threadIdx.x == 255) s.traw[blockIdx.x] = wpre + incl;
 }
 
-// Exclusive scan of the tile sums (one workgroup).  It also starts the
-// substep's bookkeeping: the per-substep counters (not the sticky overflow
-// flag) are zeroed here, before the scatter counts the active blocks.
+// Exclusive scan of the tile sums (one workgroup), for more than
+// kScatterScanTiles tiles, with the substep's bookkeeping.
 __global__ __launch_bounds__(256) void k_scan_tsum(SortDev s, uint32_t ntiles) {
   __shared__ uint32_t wtot[4];
   __shared__ uint32_t carry_s;
   if (threadIdx.x == 0) carry_s = 0;
-  if (threadIdx.x < kCntOverflow) s.counters[threadIdx.x] = 0u;
-  // The count kernel's wave runs: total to the host-mapped word, counters
-  // cleared for the next substep.
-  if (threadIdx.x < 64) {
-    uint32_t r = s.runs[threadIdx.x * kRunStride];
-    s.runs[threadIdx.x * kRunStride] = 0u;
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) r += __shfl_down(r, d, 64);
-    if (threadIdx.x == 0 && s.runs_host) *(volatile uint32_t*)s.runs_host = r;
-  }
+  start_substep(s);
   __syncthreads();
   for (uint32_t c = 0; c < ntiles; c += 256) {
     const uint32_t i = c + threadIdx.x;
-    const uint32_t v = i < ntiles ? s.tsum[i] : 0u;
+    const uint32_t v = i < ntiles ? s.traw[i] : 0u;
     const uint32_t incl = wave_incl_scan(v);
     const int wave = threadIdx.x >> 6, lane = __lane_id();
     if (lane == 63) wtot[wave] = incl;
@@ -300,9 +311,11 @@ __global__ __launch_bounds__(256) void k_scan_tsum(SortDev s, uint32_t ntiles) {
 // activation flags into the active-block list (one atomic per wave; flags
 // cleared for the next substep) and the zeroing of each listed block's
 // accumulators, which the wave does one block at a time with coalesced
-// stores.  All of it only needs the finished scans.
+// stores.  All of it only needs the finished scans.  With at most
+// kScatterScanTiles tiles every wave scans the tile sums itself (and the
+// first wave stores the scan for the later kernels).
 __global__ __launch_bounds__(256) void k_scatter(uint32_t n, uint32_t nb, SortDev s, GridDev g,
-                                                 int acc) {
+                                                 int acc, uint32_t ntiles) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   const bool on = i < nb && s.flag[i] != 0u;
   const unsigned long long m = __ballot(on);
@@ -328,10 +341,25 @@ __global__ __launch_bounds__(256) void k_scatter(uint32_t n, uint32_t nb, SortDe
       }
     }
   }
+  // key, rank and block offset first, so the tile prefix below does not
+  // add a round trip to memory
+  uint32_t k = 0, r = 0, o = 0;
   if (i < n) {
-    const uint32_t k = s.key[i];
-    s.perm[s.off[k] + s.tsum[k / kScanTile] + s.rank[i]] = i;
+    k = s.key[i];
+    r = s.rank[i];
+    o = s.off[k];
   }
+  uint32_t tpre;
+  if (ntiles <= kScatterScanTiles) {  // uniform; every lane active here
+    const int lane = __lane_id();
+    const uint32_t v = lane < (int)ntiles ? s.traw[lane] : 0u;
+    const uint32_t excl = wave_incl_scan(v) - v;
+    if (blockIdx.x == 0 && threadIdx.x < ntiles) s.tsum[threadIdx.x] = excl;
+    tpre = (uint32_t)__shfl((int)excl, (int)(k / kScanTile), 64);
+  } else {
+    tpre = i < n ? s.tsum[k / kScanTile] : 0u;
+  }
+  if (i < n) s.perm[o + tpre + r] = i;
 }
 
 // Zero the accumulators of this substep's active blocks (one wave per block).
@@ -497,12 +525,14 @@ void launch_count(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, uint32_t
 }
 void launch_scan(hipStream_t st, const SceneDev* sc, int ns, SortDev s, uint32_t nb) {
   const uint32_t tiles = (nb + kScanTile - 1) / kScanTile;
-  launch_k(k_scan_tiles, tiles, 256, 0, st, sc, ns, s, nb);
-  launch_k(k_scan_tsum, 1, 256, 0, st, s, tiles);
+  const int book = tiles <= kScatterScanTiles ? 1 : 0;
+  launch_k(k_scan_tiles, tiles ? tiles : 1u, 256, 0, st, sc, ns, s, nb, book);
+  if (!book) launch_k(k_scan_tsum, 1, 256, 0, st, s, tiles);
 }
 void launch_scatter(hipStream_t st, uint32_t n, uint32_t nb, SortDev s, GridDev g, int acc) {
   const uint32_t m = n > nb ? n : nb;
-  if (m) launch_k(k_scatter, grid_for(m, 256), 256, 0, st, n, nb, s, g, acc);
+  const uint32_t tiles = (nb + kScanTile - 1) / kScanTile;
+  if (m) launch_k(k_scatter, grid_for(m, 256), 256, 0, st, n, nb, s, g, acc, tiles);
 }
 void launch_clear(hipStream_t st, SortDev s, GridDev g, int acc, int upd_wgs) {
   k_clear<<<upd_wgs, 256, 0, st>>>(s, g, acc);
