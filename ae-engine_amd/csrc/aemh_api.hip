@@ -1547,15 +1547,17 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   // where a workgroup's particles span few blocks: a smaller tile budget
   // that wins there sends particles to the slow global path once the scene
   // spreads (1M snow: 3 tiles won at the tuning step, then took 106 us per
-  // P2G against 82 us with 4).  FP32 tiles (4 sums, 6.9 KB) never limit
-  // occupancy, so that mode keeps the full budget; the deterministic
-  // mode's (8 sums, 13.8 KB) do, and there each tile fewer than the most is
-  // charged 3 %.
+  // P2G against 82 us with 4).  Five FP32 tiles (4 sums, 6.9 KB each) still
+  // leave LDS for 4 workgroups per CU, the VGPR limit, so that mode keeps the
+  // full budget (1M snow P2G 81.5 -> 79 us against 4 tiles; 6 tiles cost a
+  // workgroup per CU and 3 %).  The deterministic mode's tiles (8 sums,
+  // 13.8 KB) do limit occupancy: it tries 2 to 4, each tile below 4 charged
+  // 3 %.
   s = ensure_partials(c);
   if (s != AEMH_OK) return s;
   for (int pairs = 0; pairs <= (q32 ? 0 : 1); pairs++)
   for (int part = 0; part <= 1; part++)
-    for (int tiles = q32 ? 2 : kMaxTileBudget; tiles <= kMaxTileBudget; tiles++) {
+    for (int tiles = q32 ? 2 : kMaxTileBudget; tiles <= (q32 ? 4 : kMaxTileBudget); tiles++) {
       ls.p2g_tiles = tiles;
       ls.p2g_partials = part;
       ls.p2g_pairs = pairs;
@@ -1570,7 +1572,7 @@ aemh_status aemh_autotune(aemh_ctx* c) {
         }
       });
       if (s != AEMH_OK) return s;
-      const float score = ms * (1.0f + 0.03f * (float)(kMaxTileBudget - tiles));
+      const float score = ms * (1.0f + 0.03f * (float)((q32 ? 4 : kMaxTileBudget) - tiles));
       if (score < best) best = score, best_ms = ms, pick = tiles, pick_part = part, pick_pairs = pairs;
     }
   ls.p2g_tiles = pick;
@@ -1638,7 +1640,8 @@ aemh_status aemh_get_launch_shape(aemh_ctx* c, uint32_t* p2g_tiles, uint32_t* p2
 aemh_status aemh_set_launch_shape(aemh_ctx* c, uint32_t p2g_tiles, uint32_t p2g_partials,
                                   uint32_t g2p_wg, uint32_t upd_wgs) {
   if (!c) return AEMH_ERR_ARG;
-  if (p2g_tiles < 2 || p2g_tiles > 4) return fail(c, AEMH_ERR_ARG, "p2g_tiles must be 2, 3 or 4");
+  if (p2g_tiles < 2 || p2g_tiles > (uint32_t)kMaxTileBudget)
+    return fail(c, AEMH_ERR_ARG, "p2g_tiles must be 2 to %d", kMaxTileBudget);
   if (p2g_partials > 3)
     return fail(c, AEMH_ERR_ARG, "p2g_partials must be 0-3 (bit 0 partial flush, bit 1 pairs)");
   if (g2p_wg != 64 && g2p_wg != 128 && g2p_wg != 256 && g2p_wg != 512)

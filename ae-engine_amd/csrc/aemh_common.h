@@ -21,7 +21,7 @@ constexpr int kBlockNodes = 64;
 // P2G stages the 6x6x6 node halo of a 4x4x4 block of base cells in LDS.
 constexpr int kTileSide = 6;
 constexpr int kTileNodes = 216;
-constexpr int kMaxTileBudget = 4;  // most LDS tiles a P2G workgroup may use
+constexpr int kMaxTileBudget = 5;  // most LDS tiles a P2G workgroup may use
 
 // One scene as seen by the kernels (derived from aemh_scene + aemh_config).
 struct SceneDev {

@@ -38,7 +38,7 @@ inline void launch_k(void (*k)(KArgs...), dim3 grid, dim3 block, size_t shmem, h
 // Launch parameters the autotuner chooses (defaults: the 1M-particle snow
 // workload's best).  P2G always runs 256 particles per workgroup.
 struct LaunchShape {
-  int p2g_tiles = 3;     // LDS tiles per P2G workgroup: 2, 3 or 4
+  int p2g_tiles = 3;     // LDS tiles per P2G workgroup: 2 to 5
   int p2g_partials = 0;  // P2G tile flush: 0 one HBM atomic per node value,
                          // 1 partial stores gathered by the grid update
   int p2g_pairs = 0;     // FP32 mode: 1 pairs lanes of one base cell and adds

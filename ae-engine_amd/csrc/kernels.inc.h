@@ -1621,8 +1621,10 @@ void launch_p2g_store(hipStream_t st, int tiles, int partials, unsigned grid, PB
     launch_k(k_p2g<STORE, 2, PAIRS>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
   else if (tiles == 3)
     launch_k(k_p2g<STORE, 3, PAIRS>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
-  else
+  else if (tiles == 4)
     launch_k(k_p2g<STORE, 4, PAIRS>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
+  else
+    launch_k(k_p2g<STORE, 5, PAIRS>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
 }
 
 template <bool PAIRS>

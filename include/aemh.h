@@ -275,7 +275,7 @@ aemh_status aemh_set_scenes(aemh_ctx* ctx, const aemh_scene* scenes, uint32_t n)
 aemh_status aemh_batch_step(aemh_ctx* ctx, uint32_t n_substeps);
 /* Times candidate launch shapes on the current particles and scenes and
  * keeps the fastest of each: P2G LDS tiles per workgroup (deterministic
- * mode: 2/3/4, a budget below 4 must beat 4 by 3 % per tile; FP32 mode: 4)
+ * mode: 2/3/4, a budget below 4 must beat 4 by 3 % per tile; FP32 mode: 5)
  * together
  * with the tile flush (one HBM atomic per node value, or partial sums stored
  * per tile and added by the grid update) and, in the FP32 mode, the pair
@@ -292,7 +292,8 @@ aemh_status aemh_autotune(aemh_ctx* ctx);
 aemh_status aemh_get_launch_shape(aemh_ctx* ctx, uint32_t* p2g_tiles, uint32_t* p2g_partials,
                                   uint32_t* g2p_wg, uint32_t* upd_wgs, float* tuned_ms3);
 /* Sets the launch shape directly (results do not depend on it, bit for bit
- * in the deterministic mode; p2g_partials 0-3 as above). */
+ * in the deterministic mode; p2g_tiles 2-5, default 3; p2g_partials 0-3 as
+ * above). */
 aemh_status aemh_set_launch_shape(aemh_ctx* ctx, uint32_t p2g_tiles, uint32_t p2g_partials,
                                   uint32_t g2p_wg, uint32_t upd_wgs);
 
