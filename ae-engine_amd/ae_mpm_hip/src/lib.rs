@@ -243,7 +243,8 @@ pub mod ffi {
         pub fn aemh_get_launch_shape(ctx: *mut AemhCtx, p2g_tiles: *mut u32,
                                      p2g_partials: *mut u32, g2p_wg: *mut u32,
                                      upd_wgs: *mut u32, tuned_ms3: *mut f32) -> c_int;
-        // p2g_partials: bit 0 partial-sum tile flush, bit 1 pair reduction (FP32 mode)
+        // p2g_partials: bit 0 partial-sum tile flush, bit 1 pair reduction (FP32 mode),
+        // bit 2 lane layers
         pub fn aemh_set_launch_shape(ctx: *mut AemhCtx, p2g_tiles: u32, p2g_partials: u32,
                                      g2p_wg: u32, upd_wgs: u32) -> c_int;
 

@@ -445,21 +445,22 @@ class Engine:
         return self.launch_shape()
 
     def launch_shape(self):
-        """dict(p2g_tiles, p2g_partials, p2g_pairs, g2p_wg, upd_wgs,
-        tuned_ms=(p2g + update, update, g2p)); the ABI carries p2g_pairs as
-        bit 1 of p2g_partials."""
+        """dict(p2g_tiles, p2g_partials, p2g_pairs, p2g_layers, g2p_wg,
+        upd_wgs, tuned_ms=(p2g + update, update, g2p)); the ABI carries
+        p2g_pairs and p2g_layers as bits 1 and 2 of p2g_partials."""
         a, p, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
         ms = np.zeros(3, np.float32)
         self._check(self._lib.aemh_get_launch_shape(self._h, C.byref(a), C.byref(p), C.byref(b),
                                                     C.byref(c), _f(ms)))
-        return dict(p2g_tiles=a.value, p2g_partials=p.value & 1, p2g_pairs=p.value >> 1,
+        return dict(p2g_tiles=a.value, p2g_partials=p.value & 1, p2g_pairs=(p.value >> 1) & 1,
+                    p2g_layers=(p.value >> 2) & 1,
                     g2p_wg=b.value, upd_wgs=c.value, tuned_ms=tuple(float(x) for x in ms))
 
     def set_launch_shape(self, p2g_tiles=3, p2g_partials=1, g2p_wg=256, upd_wgs=2048,
-                         p2g_pairs=0):
-        self._check(self._lib.aemh_set_launch_shape(self._h, p2g_tiles,
-                                                    p2g_partials | (p2g_pairs << 1), g2p_wg,
-                                                    upd_wgs))
+                         p2g_pairs=0, p2g_layers=0):
+        self._check(self._lib.aemh_set_launch_shape(
+            self._h, p2g_tiles, p2g_partials | (p2g_pairs << 1) | (p2g_layers << 2), g2p_wg,
+            upd_wgs))
 
     # -- readback ---------------------------------------------------------------
     def download_grid(self, scene=0, nodes=None):

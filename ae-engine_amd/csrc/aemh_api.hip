@@ -1540,10 +1540,16 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   CHECK_LAUNCH(c);
   constexpr int kReps = 3;
   float best = 1e30f, best_ms = 0.0f, ms = 0.0f;
-  int pick = ls.p2g_tiles, pick_part = ls.p2g_partials, pick_pairs = ls.p2g_pairs;
-  // Tile budget and flush mode together: the flush mode moves work between
-  // P2G and the grid update, so both are timed.  The FP32 mode also times
-  // the pair reduction.  The state being timed is often freshly sorted,
+  int pick = ls.p2g_tiles, pick_part = ls.p2g_partials, pick_pairs = ls.p2g_pairs,
+      pick_layers = ls.p2g_layers;
+  // Tile budget, flush mode and lane assignment together: the flush mode
+  // moves work between P2G and the grid update, so both are timed.  Lanes
+  // keep their particles, or are re-assigned by rank within their cell
+  // (layers).  The pair reduction is not a candidate: it can win on the
+  // tuning state and then loses over the run on every workload measured
+  // (1M snow 91.6 against 77.3 us, 16M 1.16 against 0.89 ms, the 256-scene
+  // batch 1.43 against 1.13 ms per P2G); aemh_set_launch_shape selects it.
+  // The state being timed is often freshly sorted,
   // where a workgroup's particles span few blocks: a smaller tile budget
   // that wins there sends particles to the slow global path once the scene
   // spreads (1M snow: 3 tiles won at the tuning step, then took 106 us per
@@ -1555,12 +1561,14 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   // 3 %.
   s = ensure_partials(c);
   if (s != AEMH_OK) return s;
-  for (int pairs = 0; pairs <= (q32 ? 0 : 1); pairs++)
+  for (int layers = 0; layers <= 1; layers++)
   for (int part = 0; part <= 1; part++)
     for (int tiles = q32 ? 2 : kMaxTileBudget; tiles <= (q32 ? 4 : kMaxTileBudget); tiles++) {
+      const int pairs = 0;
       ls.p2g_tiles = tiles;
       ls.p2g_partials = part;
       ls.p2g_pairs = pairs;
+      ls.p2g_layers = layers;
       s = time_launches(c, kReps, &ms, [&] {
         launch_clear(st, c->sort, c->grid, acc_layout(c), ls.upd_wgs);
         if (q32) {
@@ -1573,11 +1581,13 @@ aemh_status aemh_autotune(aemh_ctx* c) {
       });
       if (s != AEMH_OK) return s;
       const float score = ms * (1.0f + 0.03f * (float)((q32 ? 4 : kMaxTileBudget) - tiles));
-      if (score < best) best = score, best_ms = ms, pick = tiles, pick_part = part, pick_pairs = pairs;
+      if (score < best)
+        best = score, best_ms = ms, pick = tiles, pick_part = part, pick_pairs = pairs, pick_layers = layers;
     }
   ls.p2g_tiles = pick;
   ls.p2g_partials = pick_part;
   ls.p2g_pairs = pick_pairs;
+  ls.p2g_layers = pick_layers;
   c->tune_ms[0] = best_ms;
   // Partials in the chosen tile layout for the grid-update candidates.
   launch_clear(st, c->sort, c->grid, acc_layout(c), ls.upd_wgs);
@@ -1635,7 +1645,8 @@ aemh_status aemh_get_launch_shape(aemh_ctx* c, uint32_t* p2g_tiles, uint32_t* p2
                                   uint32_t* g2p_wg, uint32_t* upd_wgs, float* tuned_ms3) {
   if (!c) return AEMH_ERR_ARG;
   if (p2g_tiles) *p2g_tiles = (uint32_t)c->shape.p2g_tiles;
-  if (p2g_partials) *p2g_partials = (uint32_t)(c->shape.p2g_partials | (c->shape.p2g_pairs << 1));
+  if (p2g_partials)
+    *p2g_partials = (uint32_t)(c->shape.p2g_partials | (c->shape.p2g_pairs << 1) | (c->shape.p2g_layers << 2));
   if (g2p_wg) *g2p_wg = (uint32_t)c->shape.g2p_wg;
   if (upd_wgs) *upd_wgs = (uint32_t)c->shape.upd_wgs;
   if (tuned_ms3)
@@ -1648,14 +1659,15 @@ aemh_status aemh_set_launch_shape(aemh_ctx* c, uint32_t p2g_tiles, uint32_t p2g_
   if (!c) return AEMH_ERR_ARG;
   if (p2g_tiles < 2 || p2g_tiles > (uint32_t)kMaxTileBudget)
     return fail(c, AEMH_ERR_ARG, "p2g_tiles must be 2 to %d", kMaxTileBudget);
-  if (p2g_partials > 3)
-    return fail(c, AEMH_ERR_ARG, "p2g_partials must be 0-3 (bit 0 partial flush, bit 1 pairs)");
+  if (p2g_partials > 7)
+    return fail(c, AEMH_ERR_ARG, "p2g_partials must be 0-7 (bit 0 partial flush, bit 1 pairs, bit 2 layers)");
   if (g2p_wg != 64 && g2p_wg != 128 && g2p_wg != 256 && g2p_wg != 512)
     return fail(c, AEMH_ERR_ARG, "g2p_wg must be 64, 128, 256 or 512");
   if (upd_wgs < 1 || upd_wgs > 65536) return fail(c, AEMH_ERR_ARG, "upd_wgs must be in [1, 65536]");
   c->shape.p2g_tiles = (int)p2g_tiles;
   c->shape.p2g_partials = (int)(p2g_partials & 1u);
-  c->shape.p2g_pairs = (int)(p2g_partials >> 1);
+  c->shape.p2g_pairs = (int)((p2g_partials >> 1) & 1u);
+  c->shape.p2g_layers = (int)((p2g_partials >> 2) & 1u);
   c->shape.g2p_wg = (int)g2p_wg;
   c->shape.upd_wgs = (int)upd_wgs;
   c->tuned = true;

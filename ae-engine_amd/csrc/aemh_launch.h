@@ -43,6 +43,8 @@ struct LaunchShape {
                          // 1 partial stores gathered by the grid update
   int p2g_pairs = 0;     // FP32 mode: 1 pairs lanes of one base cell and adds
                          // their node values across the pair before the LDS atomics
+  int p2g_layers = 0;    // 1 re-assigns lanes by rank within their base cell
+                         // so one LDS atomic's lanes hit distinct nodes
   int g2p_wg = 256;      // G2P workgroup size: 64, 128, 256 or 512
   int upd_wgs = 2048;    // persistent grid-update / clear workgroups
 };

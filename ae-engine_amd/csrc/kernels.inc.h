@@ -48,8 +48,9 @@ constexpr int kTileVals = kTileFields * kTileNodes;
 // Values per tile-node record of the P2G partials (the 8 deterministic-mode
 // fields, or the 4 FP32 sums).
 constexpr int kPartStride = AEMH_DET ? 8 : 4;
-// Lane counters of the pair-reduced P2G's re-assignment (kPairLayers pair
-// layers, then the unpaired particles and those without a tile).
+// Lane layers of the P2G re-assignment (rank within a base cell, capped),
+// and the lane counters of the pair-reduced form (kPairLayers pair layers,
+// then the unpaired particles and those without a tile).
 constexpr int kLayers = 16;
 // Pair layers of the pair-reduced P2G (up to 8 pairs per base cell).
 constexpr int kPairLayers = 8;
@@ -384,7 +385,8 @@ __device__ __forceinline__ void tile_node_of(int t, int& lx, int& ly, int& lz) {
 template <int STORE, int TILES, bool PAIRS = false>
 __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict__ sc, int ns,
                                              const MatDev* __restrict__ mats, int nmat,
-                                             uint32_t n, SortDev s, GridDev G, int partials) {
+                                             uint32_t n, SortDev s, GridDev G, int partials,
+                                             int layers) {
   __shared__ long long tile[TILES * kTileVals];
   // pair mode: lane index of each (cell, pair layer)'s first member
   __shared__ uint8_t pbase[PAIRS ? TILES * 64 * kPairLayers : 1];
@@ -458,15 +460,32 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
     sslot[t] = (int16_t)min(slot, 32767);
     __syncthreads();
 
-    // Without the pair layers, lanes keep their particles: LDS atomics of one
-    // wave that hit the same node (particles of one base cell, about 2 per
-    // cell at 1M particles and 5 at 16M) cost less than re-assigning lanes by
-    // rank within their cell so that they never do, which took an LDS
-    // counting pass and a barrier (1M snow P2G 85.6 -> 81.3 us, 16M 0.982 ->
-    // 0.877 ms).  The variant that also dropped the order array, the cell
-    // counters and the second scene lookup measured 3 % slower on both.
+    // ---- layers (an autotuned option): rank of the particle among the
+    // workgroup's particles of its base cell.  Lanes are re-assigned layer by
+    // layer, so the 64 lanes of one LDS atomic hit distinct nodes (the first
+    // kLayers - 1 layers).  Without them lanes keep their particles, and LDS
+    // atomics of one wave that hit the same node are serialized: cheaper
+    // where the values to add are few and the particles per cell moderate
+    // (FP32, 1M snow P2G 85.6 -> 81.3 us, 16M 0.982 -> 0.877 ms), dearer for
+    // the deterministic mode's 8 sums per node, the velocity-only tier's
+    // cheap contributions and dense batched scenes. ----
     if constexpr (!PAIRS) {
-      order[t] = (uint8_t)t;
+      if (layers) {  // uniform
+        int layer = kLayers - 1;
+        if (valid && slot < TILES) {
+          int c = 0;
+#pragma unroll
+          for (int a = 2; a >= 0; a--) c = c * 4 + min(max(st.g[a] - torg[slot][a], 0), 3);
+          layer = (int)min(atomicAdd(&ccnt[slot * 64 + c], 1u), (uint32_t)(kLayers - 1));
+        }
+        const uint32_t rl = atomicAdd(&lcnt[layer], 1u);
+        __syncthreads();
+        uint32_t off = 0;
+        for (int l = 0; l < layer; l++) off += lcnt[l];
+        order[off + rl] = (uint8_t)t;
+      } else {
+        order[t] = (uint8_t)t;
+      }
     } else {
       // ---- pair layers: two particles of one base cell go to lanes 2i and
       // 2i + 1, which add their node values across lanes before the LDS
@@ -1614,17 +1633,17 @@ void launch_p2g_serial(hipStream_t st, PBuf cur, const SceneDev* sc, int ns, con
 #endif
 
 template <int STORE, bool PAIRS>
-void launch_p2g_store(hipStream_t st, int tiles, int partials, unsigned grid, PBuf cur,
+void launch_p2g_store(hipStream_t st, int tiles, int partials, int layers, unsigned grid, PBuf cur,
                       const SceneDev* sc, int ns, const MatDev* mats, int nmat, uint32_t n,
                       SortDev s, GridDev g) {
   if (tiles <= 2)
-    launch_k(k_p2g<STORE, 2, PAIRS>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
+    launch_k(k_p2g<STORE, 2, PAIRS>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials, layers);
   else if (tiles == 3)
-    launch_k(k_p2g<STORE, 3, PAIRS>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
+    launch_k(k_p2g<STORE, 3, PAIRS>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials, layers);
   else if (tiles == 4)
-    launch_k(k_p2g<STORE, 4, PAIRS>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
+    launch_k(k_p2g<STORE, 4, PAIRS>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials, layers);
   else
-    launch_k(k_p2g<STORE, 5, PAIRS>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials);
+    launch_k(k_p2g<STORE, 5, PAIRS>, grid, 256, 0, st, cur, sc, ns, mats, nmat, n, s, g, partials, layers);
 }
 
 template <bool PAIRS>
@@ -1632,11 +1651,11 @@ void launch_p2g_pairs(hipStream_t st, const LaunchShape& ls, PBuf cur, const Sce
                       const MatDev* mats, int nmat, uint32_t n, SortDev s, GridDev g, int storage) {
   const unsigned grid = (n + 255u) / 256u;
   if (storage == 1)
-    launch_p2g_store<1, PAIRS>(st, ls.p2g_tiles, ls.p2g_partials, grid, cur, sc, ns, mats, nmat, n, s, g);
+    launch_p2g_store<1, PAIRS>(st, ls.p2g_tiles, ls.p2g_partials, ls.p2g_layers, grid, cur, sc, ns, mats, nmat, n, s, g);
   else if (storage == 2)
-    launch_p2g_store<2, PAIRS>(st, ls.p2g_tiles, ls.p2g_partials, grid, cur, sc, ns, mats, nmat, n, s, g);
+    launch_p2g_store<2, PAIRS>(st, ls.p2g_tiles, ls.p2g_partials, ls.p2g_layers, grid, cur, sc, ns, mats, nmat, n, s, g);
   else
-    launch_p2g_store<0, PAIRS>(st, ls.p2g_tiles, ls.p2g_partials, grid, cur, sc, ns, mats, nmat, n, s, g);
+    launch_p2g_store<0, PAIRS>(st, ls.p2g_tiles, ls.p2g_partials, ls.p2g_layers, grid, cur, sc, ns, mats, nmat, n, s, g);
 }
 
 void launch_p2g(hipStream_t st, const LaunchShape& ls, PBuf cur, const SceneDev* sc, int ns,

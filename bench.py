@@ -56,17 +56,22 @@ KERNELS = ["count", "scan", "scatter", "p2g", "grid_update", "g2p"]
 # HBM bytes per dispatch from rocprofv3 FETCH_SIZE (x 2.000) and WRITE_SIZE
 # (x 1.000), calibrated against a 256 MiB copy, as recorded in
 # profiles/p2g_g2p_sq_counters.md (1M-particle snow workload, launch shape
-# 4,1,256,8192; 16M-particle workload, early steps, 4,0,256,8192).
-TRAFFIC_C2 = {"p2g": 189.9e6, "g2p": 220.4e6, "grid_update": 90.8e6}
-TRAFFIC_C3 = {"p2g": 2578e6, "g2p": 3068e6}
+# 5,1,256,8192; 16M-particle workload, early steps, 5,0,256,8192).
+TRAFFIC_C2 = {"p2g": 188.1e6, "g2p": 220.2e6, "grid_update": 90.9e6}
+TRAFFIC_C3 = {"p2g": 2575e6, "g2p": 3068e6}
 # VALU instructions per wave (one wave = 64 particles) from rocprofv3
 # SQ_INSTS_VALU / SQ_WAVES, recorded in profiles/p2g_g2p_sq_counters.md; P2G
-# without and with the pair reduction.  The issue ceiling without packed
-# FP32: one wave64 VALU instruction per 4 cycles per SIMD, 4 SIMDs x 256 CUs
-# at 2.4 GHz = 614.4e9 wave-instructions/s.
+# by its lane assignment (lanes keep their particles, layers, pairs).  The
+# issue ceiling without packed FP32: one wave64 VALU instruction per 4 cycles
+# per SIMD, 4 SIMDs x 256 CUs at 2.4 GHz = 614.4e9 wave-instructions/s.
 VALU_PEAK_WINST = 256 * 4 * 2.4e9 / 4
-VALU_PER_WAVE_C2 = {"p2g": (1581, 1737), "g2p": 1521}
-VALU_PER_WAVE_C3 = {"p2g": (1594, 1790), "g2p": 1294}
+VALU_PER_WAVE_C2 = {"p2g": {"own": 1533, "layers": None, "pairs": 1743}, "g2p": 1521}
+VALU_PER_WAVE_C3 = {"p2g": {"own": 1539, "layers": None, "pairs": 1800}, "g2p": 1296}
+
+
+def lanes_of(shape):
+    """The P2G lane assignment of a launch shape."""
+    return "pairs" if shape["p2g_pairs"] else ("layers" if shape["p2g_layers"] else "own")
 
 
 def log(msg):
@@ -103,7 +108,7 @@ def restore(eng, wl, snap, shape):
     eng.upload(snap["pos"], vel=snap["vel"], C9=snap["C"], F9=snap["F"], J=snap["J"],
                mass=wl.mass, mat=wl.mat, Jp=snap["Jp"])
     eng.set_launch_shape(shape["p2g_tiles"], shape["p2g_partials"], shape["g2p_wg"],
-                         shape["upd_wgs"], shape["p2g_pairs"])
+                         shape["upd_wgs"], shape["p2g_pairs"], shape["p2g_layers"])
 
 
 def run_main(W, aemh, steps, warmup, accum=None):
@@ -141,12 +146,12 @@ def run_main(W, aemh, steps, warmup, accum=None):
     return wl, el, el_sync, el_prof, kern, blocks, npart, shape
 
 
-def roofline_of(kern, name, npart, blocks, traffic=None, valu=None, pairs=0):
+def roofline_of(kern, name, npart, blocks, traffic=None, valu=None, lanes="own"):
     """HBM roofline of one kernel: algorithmic bytes per launch over its average
     HIP-event duration.  With `valu` (VALU instructions per wave from the
-    counter profile; P2G's by its pair reduction flag), also the VALU issue
-    rate against its ceiling: P2G is limited by VALU issue and LDS atomics
-    long before HBM.  A G2P that counted the next substep's sort (no count
+    counter profile; P2G's by its lane assignment), also the VALU issue rate
+    against its ceiling: P2G is limited by VALU issue and LDS atomics long
+    before HBM.  A G2P that counted the next substep's sort (no count
     kernel in the window) is charged its key and rank writes."""
     counted = name == "g2p" and kern.get("count", 0.0) == 0.0
     per_p, per_n = BYTES["g2p_count" if counted else name]
@@ -157,8 +162,10 @@ def roofline_of(kern, name, npart, blocks, traffic=None, valu=None, pairs=0):
            "frac": round(achieved / HBM_PEAK_GBS, 4),
            "traffic": None if traffic is None else int(traffic.get(name, 0)) or None,
            "kernel": name, "algorithmic_bytes": int(nbytes), "avg_ms": round(ms, 4)}
-    if valu and name in valu and ms > 0:
-        per_wave = valu[name][1 if pairs else 0] if isinstance(valu[name], tuple) else valu[name]
+    per_wave = None
+    if valu and name in valu:
+        per_wave = valu[name].get(lanes) if isinstance(valu[name], dict) else valu[name]
+    if per_wave and ms > 0:
         rate = (npart + 63) // 64 * per_wave / (ms * 1e-3)
         out["valu_issue"] = {"insts_per_wave": per_wave, "achieved": round(rate / 1e9, 1),
                              "peak": VALU_PEAK_WINST / 1e9, "unit": "G wave-instructions/s",
@@ -280,7 +287,7 @@ def extra_c3(W, aemh, steps=96):
         "step_hbm_frac": round(step_bytes / (el / steps) / 1e9 / HBM_PEAK_GBS, 4),
         "g2p": roofline_of(kern, "g2p", npart, blocks, TRAFFIC_C3, VALU_PER_WAVE_C3),
         "p2g": roofline_of(kern, "p2g", npart, blocks, TRAFFIC_C3, VALU_PER_WAVE_C3,
-                           shape["p2g_pairs"]),
+                           lanes_of(shape)),
         "ms_by_window": windows,
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
         "launch_shape": shape,
@@ -626,7 +633,7 @@ def main():
                          "10 steps of the FixedPoint cloud (fixed)")
     ap.add_argument("--shape", default=None,
                     help="with --probe: a fixed launch shape 'p2g_tiles,p2g_partials,g2p_wg,"
-                         "upd_wgs[,p2g_pairs]' instead of the autotuner's pick (the autotuner's "
+                         "upd_wgs[,p2g_pairs[,p2g_layers]]' instead of the autotuner's pick (the autotuner's "
                          "timings move under a profiler)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -663,9 +670,10 @@ def main():
                                "128^3 grid, dx 1/128, dt 5e-5, FP32 P2G",
                    "particles": wl.n, "grid": [128, 128, 128], "accum": "f32", "substeps": 1,
                    "launch_shape": {k: shape[k] for k in ("p2g_tiles", "p2g_partials", "p2g_pairs",
+                                                            "p2g_layers",
                                                             "g2p_wg", "upd_wgs")}},
         "roofline": roofline_of(kern, dominant, npart, blocks, TRAFFIC_C2, VALU_PER_WAVE_C2,
-                                shape["p2g_pairs"]),
+                                lanes_of(shape)),
         "step_hbm_frac": round(step_bytes / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
     }

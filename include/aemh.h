@@ -276,24 +276,27 @@ aemh_status aemh_batch_step(aemh_ctx* ctx, uint32_t n_substeps);
 /* Times candidate launch shapes on the current particles and scenes and
  * keeps the fastest of each: P2G LDS tiles per workgroup (deterministic
  * mode: 2/3/4, a budget below 4 must beat 4 by 3 % per tile; FP32 mode: 5)
- * together
- * with the tile flush (one HBM atomic per node value, or partial sums stored
- * per tile and added by the grid update) and, in the FP32 mode, the pair
- * reduction (lanes holding two particles of one base cell add their node
- * values across the pair before the LDS atomics), grid-update workgroups
- * (1024..16384) and the G2P workgroup (64/128/256/512).  The particle state is
- * not advanced.  Called once by the first batch step; plain steps use the
+ * together with the tile flush (one HBM atomic per node value, or partial
+ * sums stored per tile and added by the grid update) and the lane
+ * assignment: lanes keep their sorted particles, or are re-assigned by rank
+ * within their base cell so one LDS atomic's lanes hit distinct nodes
+ * (layers); then grid-update workgroups (1024..16384) and the G2P workgroup
+ * (64/128/256/512, sizes other than 256 must win by 3 %).  The particle state
+ * is not advanced.  Called once by the first batch step; plain steps use the
  * defaults unless this is called.  No reference counterpart. */
 aemh_status aemh_autotune(aemh_ctx* ctx);
 /* The launch shape in use and, after aemh_autotune, the winning times in ms
  * (P2G + grid update, grid update, G2P; zeros before tuning).  NULL outputs
- * are skipped.  p2g_partials carries two P2G flags: bit 0 the partial-sum
- * tile flush, bit 1 the pair reduction (FP32 mode). */
+ * are skipped.  p2g_partials carries three P2G flags: bit 0 the partial-sum
+ * tile flush, bit 1 the pair reduction (FP32 mode: lanes holding two
+ * particles of one base cell add their node values across the pair before
+ * the LDS atomics; set only through aemh_set_launch_shape), bit 2 the
+ * layers. */
 aemh_status aemh_get_launch_shape(aemh_ctx* ctx, uint32_t* p2g_tiles, uint32_t* p2g_partials,
                                   uint32_t* g2p_wg, uint32_t* upd_wgs, float* tuned_ms3);
 /* Sets the launch shape directly (results do not depend on it, bit for bit
- * in the deterministic mode; p2g_tiles 2-5, default 3; p2g_partials 0-3 as
- * above). */
+ * in the deterministic mode; p2g_tiles 2-5, default 3; p2g_partials 0-7 as
+ * above, default 0). */
 aemh_status aemh_set_launch_shape(aemh_ctx* ctx, uint32_t p2g_tiles, uint32_t p2g_partials,
                                   uint32_t g2p_wg, uint32_t upd_wgs);
 

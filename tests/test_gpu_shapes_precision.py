@@ -58,7 +58,8 @@ def sparse_snow():
 
 @pytest.mark.parametrize("shape", [(2, 0, 64, 1024), (3, 1, 128, 2048), (4, 1, 256, 4096),
                                    (2, 1, 256, 16384), (4, 0, 128, 8192), (3, 0, 512, 2048),
-                                   (5, 1, 256, 2048), (5, 0, 64, 4096)])
+                                   (5, 1, 256, 2048), (5, 0, 64, 4096), (4, 5, 256, 2048),
+                                   (3, 4, 128, 4096)])
 def test_launch_shapes_deterministic_bitexact(sparse_snow, shape):
     wl, st = sparse_snow
     eng = aemh.Engine()
@@ -118,7 +119,7 @@ def test_profiled_steps_are_exact_and_timed():
 
 def test_launch_shape_rejects_bad_values():
     eng = aemh.Engine()
-    for bad in [(1, 1, 256, 2048), (6, 1, 256, 2048), (3, 4, 256, 2048), (3, 1, 96, 2048),
+    for bad in [(1, 1, 256, 2048), (6, 1, 256, 2048), (3, 8, 256, 2048), (3, 1, 96, 2048),
                 (3, 1, 1024, 2048), (3, 1, 256, 0)]:
         with pytest.raises(aemh.AemhError):
             eng.set_launch_shape(*bad)
@@ -128,14 +129,17 @@ def test_launch_shape_rejects_bad_values():
 def test_fp32_flush_modes_agree():
     """FP32 mode: the atomic and the partial-sum tile flush, each with and
     without the pair reduction (two particles of a base cell adding their
-    node values across lanes before the LDS atomics), with 3 and 5 LDS
-    tiles, differ only in the order and grouping of float additions."""
+    node values across lanes before the LDS atomics), with and without the
+    lane layers, with 3 and 5 LDS tiles, differ only in the order and
+    grouping of float additions."""
     wl = W.c2_snow_dam(n=1 << 16, steps=10, res=64)
     out = []
-    for part, pairs, tiles in ((0, 0, 3), (1, 0, 3), (0, 1, 3), (1, 1, 3), (1, 0, 5), (0, 1, 5)):
+    for part, pairs, tiles, layers in ((0, 0, 3, 0), (1, 0, 3, 0), (0, 1, 3, 0), (1, 1, 3, 0),
+                                       (1, 0, 5, 0), (0, 1, 5, 0), (1, 0, 5, 1), (0, 0, 3, 1)):
         eng = aemh.Engine()
         W.load_engine(eng, wl)
-        eng.set_launch_shape(tiles, part, 256, 2048, pairs)
+        eng.set_launch_shape(tiles, part, 256, 2048, pairs, layers)
+        assert eng.launch_shape()["p2g_layers"] == layers
         assert eng.launch_shape()["p2g_pairs"] == pairs
         for _ in range(10):
             eng.step(wl.dt)
