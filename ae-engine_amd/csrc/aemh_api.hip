@@ -1549,6 +1549,12 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   // tuning state and then loses over the run on every workload measured
   // (1M snow 91.6 against 77.3 us, 16M 1.16 against 0.89 ms, the 256-scene
   // batch 1.43 against 1.13 ms per P2G); aemh_set_launch_shape selects it.
+  // Where a node's values are many (the deterministic mode's 8 sums) or
+  // cheap (velocity-only scenes, most of the set), lanes hitting the same node
+  // cost more than the layering pass once particles cluster, which the
+  // freshly sorted tuning state does not show: own lanes are charged 5 %
+  // there (1M snow, deterministic mode: 1.49e9 when own lanes won the
+  // tuning, 1.63e9 with layers; tier tick 0.271 against 0.258 ms).
   // The state being timed is often freshly sorted,
   // where a workgroup's particles span few blocks: a smaller tile budget
   // that wins there sends particles to the slow global path once the scene
@@ -1561,6 +1567,10 @@ aemh_status aemh_autotune(aemh_ctx* c) {
   // 3 %.
   s = ensure_partials(c);
   if (s != AEMH_OK) return s;
+  uint64_t vel_only_n = 0;
+  for (const SceneDev& S : c->scd_h)
+    if (S.vel_only) vel_only_n += S.count;
+  const bool favor_layers = q32 || 2 * vel_only_n > (uint64_t)n;
   for (int layers = 0; layers <= 1; layers++)
   for (int part = 0; part <= 1; part++)
     for (int tiles = q32 ? 2 : kMaxTileBudget; tiles <= (q32 ? 4 : kMaxTileBudget); tiles++) {
@@ -1580,7 +1590,8 @@ aemh_status aemh_autotune(aemh_ctx* c) {
         }
       });
       if (s != AEMH_OK) return s;
-      const float score = ms * (1.0f + 0.03f * (float)((q32 ? 4 : kMaxTileBudget) - tiles));
+      const float score = ms * (1.0f + 0.03f * (float)((q32 ? 4 : kMaxTileBudget) - tiles)) *
+                          (favor_layers && !layers ? 1.05f : 1.0f);
       if (score < best)
         best = score, best_ms = ms, pick = tiles, pick_part = part, pick_pairs = pairs, pick_layers = layers;
     }

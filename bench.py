@@ -684,9 +684,11 @@ def main():
     want = set() if args.no_extras else {e.strip() for e in args.extras.split(",") if e.strip()}
     if "q32" in want:
         log("[bench] deterministic Q32.32 mode")
-        wq, elq, _, _, _, _, _, _ = run_main(W, aemh, args.steps, args.warmup,
-                                             accum=aemh.ACCUM_Q32_32)
+        wq, elq, _, _, _, _, _, shq = run_main(W, aemh, args.steps, args.warmup,
+                                               accum=aemh.ACCUM_Q32_32)
         extra["q32_particle_updates_per_s"] = wq.n * args.steps / elq
+        extra["q32_launch_shape"] = {k: shq[k] for k in ("p2g_tiles", "p2g_partials", "p2g_layers",
+                                                         "g2p_wg", "upd_wgs")}
     if "c3" in want:
         log("[bench] C3 16M particles")
         extra["c3"] = extra_c3(W, aemh)

@@ -280,7 +280,8 @@ aemh_status aemh_batch_step(aemh_ctx* ctx, uint32_t n_substeps);
  * sums stored per tile and added by the grid update) and the lane
  * assignment: lanes keep their sorted particles, or are re-assigned by rank
  * within their base cell so one LDS atomic's lanes hit distinct nodes
- * (layers); then grid-update workgroups (1024..16384) and the G2P workgroup
+ * (layers; own lanes must win by 5 % in the deterministic mode and when most
+ * particles are in velocity-only scenes); then grid-update workgroups (1024..16384) and the G2P workgroup
  * (64/128/256/512, sizes other than 256 must win by 3 %).  The particle state
  * is not advanced.  Called once by the first batch step; plain steps use the
  * defaults unless this is called.  No reference counterpart. */
