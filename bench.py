@@ -57,16 +57,16 @@ KERNELS = ["count", "scan", "scatter", "p2g", "grid_update", "g2p"]
 # (x 1.000), calibrated against a 256 MiB copy, as recorded in
 # profiles/p2g_g2p_sq_counters.md (1M-particle snow workload, launch shape
 # 5,1,256,8192; 16M-particle workload, early steps, 5,0,256,8192).
-TRAFFIC_C2 = {"p2g": 188.1e6, "g2p": 220.2e6, "grid_update": 90.9e6}
-TRAFFIC_C3 = {"p2g": 2575e6, "g2p": 3068e6}
+TRAFFIC_C2 = {"p2g": 188.4e6, "g2p": 220.3e6, "grid_update": 91.0e6}
+TRAFFIC_C3 = {"p2g": 2575e6, "g2p": 3069e6}
 # VALU instructions per wave (one wave = 64 particles) from rocprofv3
 # SQ_INSTS_VALU / SQ_WAVES, recorded in profiles/p2g_g2p_sq_counters.md; P2G
 # by its lane assignment (lanes keep their particles, layers, pairs).  The
 # issue ceiling without packed FP32: one wave64 VALU instruction per 4 cycles
 # per SIMD, 4 SIMDs x 256 CUs at 2.4 GHz = 614.4e9 wave-instructions/s.
 VALU_PEAK_WINST = 256 * 4 * 2.4e9 / 4
-VALU_PER_WAVE_C2 = {"p2g": {"own": 1533, "layers": None, "pairs": 1743}, "g2p": 1521}
-VALU_PER_WAVE_C3 = {"p2g": {"own": 1539, "layers": None, "pairs": 1800}, "g2p": 1296}
+VALU_PER_WAVE_C2 = {"p2g": {"own": 1544, "layers": 1588, "pairs": 1743}, "g2p": 1521}
+VALU_PER_WAVE_C3 = {"p2g": {"own": 1546, "layers": 1601, "pairs": 1797}, "g2p": 1295}
 
 
 def lanes_of(shape):
