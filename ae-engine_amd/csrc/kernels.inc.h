@@ -1026,6 +1026,7 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
   __shared__ uint32_t skey[kG2PMaxWG];
   __shared__ int gmeta[kG2PMaxTiles][8];  // origin x, y, z; grid n x, y, z; nb0; nb1
   __shared__ uint32_t gboff[kG2PMaxTiles];
+  __shared__ uint32_t gkey[kG2PMaxTiles];  // block key of each staged tile
   __shared__ uint32_t wcnt[kG2PMaxWG / 64];
   __shared__ int gnonfinite;  // some staged node carries a non-finite velocity
   const int max_tiles = g2p_tiles_for((int)blockDim.x);
@@ -1091,9 +1092,10 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
     if (w < wave) pre += wcnt[w];
     tot += wcnt[w];
   }
-  const int slot = (int)(pre + (uint32_t)__popcll(hb & le)) - 1;
+  const int rslot = (int)(pre + (uint32_t)__popcll(hb & le)) - 1;  // the lane's key run
   const int ntiles = (int)min(tot, (uint32_t)max_tiles);
-  if (head && slot < max_tiles) {
+  if (head && rslot < max_tiles) {
+    const int slot = rslot;
     const uint32_t lb = key - S.block_off;
     gmeta[slot][0] = (int)(lb % (uint32_t)S.nb[0]) * 4;
     gmeta[slot][1] = (int)((lb / (uint32_t)S.nb[0]) % (uint32_t)S.nb[1]) * 4;
@@ -1104,8 +1106,20 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
     gmeta[slot][6] = S.nb[0];
     gmeta[slot][7] = S.nb[1];
     gboff[slot] = S.block_off;
+    gkey[slot] = key;
   }
   __syncthreads();
+  // A run past the tile budget whose block an earlier run staged uses that
+  // tile: on in-place steps particles that crossed a block face split their
+  // block's run, so later runs repeat staged blocks.
+  int slot = rslot;
+  if (valid && rslot >= max_tiles) {
+    for (int i = 0; i < ntiles; i++)
+      if (gkey[i] == key) {
+        slot = i;
+        break;
+      }
+  }
   // Tile node li of every staged tile; outside the grid (v, m) = 0.  A node
   // with m < 1e-12, which every gather skips, is staged with v = 0, so the
   // FP32 gather's product with it is zero without a test.
