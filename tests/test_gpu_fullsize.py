@@ -148,6 +148,8 @@ def _oracle(wl):
     return R, cfg, R.Materials(p, m, e)
 
 
+# 400 oracle steps on the host: about 100 s on 16 threads, more on a busy host
+@pytest.mark.timeout(600)
 def test_c2_full_size_fp32_vs_oracle_200_steps():
     """1,048,576 snow particles, the configuration's 200 FP32 steps, against
     the oracle with the reference's >= 50,000-index structure (per-thread
