@@ -412,10 +412,18 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
     // ---- tiles: one per distinct base block, in sorted order ----
     const uint32_t j = j0 + (uint32_t)t;
     const bool valid = j < n;
+    // The block key is computed from the position as the count kernel did
+    // (sorting keeps a particle inside its scene's range, so scene_of(j) is
+    // its scene): one dependent load round (perm, then x) instead of two
+    // before the first barrier, and x is at hand for the stencil.
     uint32_t p = 0, key = 0xffffffffu;
+    Stencil st = {{0, 0, 0}, {0.f, 0.f, 0.f}};
+    int sid = 0;
     if (valid) {
       p = s.perm[j];
-      key = s.key[p];
+      sid = scene_of(sc, ns, j);
+      st = make_stencil(sc[sid], P.x[0][p], P.x[1][p], P.x[2][p]);
+      key = block_key(sc[sid], st);
     }
     skey[t] = key;
     sperm[t] = p;
@@ -435,12 +443,6 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
     }
     slot = (int)(pre + (uint32_t)__popcll(hb & le)) - 1;
     ntiles = (int)min(tot, (uint32_t)TILES);
-    Stencil st = {{0, 0, 0}, {0.f, 0.f, 0.f}};
-    int sid = 0;
-    if (valid) {
-      sid = scene_of(sc, ns, j);
-      st = make_stencil(sc[sid], P.x[0][p], P.x[1][p], P.x[2][p]);
-    }
     if (partials && head) {
       // The workgroup holding a block's first particle records the block's
       // tile slot; later workgroups of the same block hold it in slot 0.
