@@ -408,6 +408,10 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
   const int lane = __lane_id(), wave = t >> 6;
   const uint32_t j0 = blockIdx.x * 256u;
   int slot, ntiles;
+  // the lane's own particle, stencil and scene (kept when lanes keep their particles)
+  uint32_t p_own = 0;
+  Stencil st_own = {{0, 0, 0}, {0.f, 0.f, 0.f}};
+  int sid_own = 0;
   {
     // ---- tiles: one per distinct base block, in sorted order ----
     const uint32_t j = j0 + (uint32_t)t;
@@ -425,6 +429,9 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
       st = make_stencil(sc[sid], P.x[0][p], P.x[1][p], P.x[2][p]);
       key = block_key(sc[sid], st);
     }
+    p_own = p;
+    st_own = st;
+    sid_own = sid;
     skey[t] = key;
     sperm[t] = p;
     if (t < kLayers) lcnt[t] = 0;
@@ -534,22 +541,31 @@ __global__ __launch_bounds__(256) void k_p2g(PBuf P, const SceneDev* __restrict_
       order[off + idx] = (uint8_t)t;
     }
     for (int e = t; e < ntiles * kTileVals; e += 256) tile[e] = 0;
-    __syncthreads();
+    // The cleared tiles are next touched after the block-exponent barrier;
+    // only re-assigned lanes need this one, for order[].
+    if (PAIRS || layers) __syncthreads();
   }
   // This lane now carries workgroup particle u (sorted position j0 + u).
-  const int u = order[t];
+  const bool own = !PAIRS && !layers;  // uniform
+  const int u = own ? t : order[t];
   const uint32_t j = j0 + (uint32_t)u;
   const bool valid = j < n;
-  const uint32_t p = sperm[u];
-  slot = sslot[u];
+  const uint32_t p = own ? p_own : sperm[u];
+  if (!own) slot = sslot[u];
 
   int sid = 0;
   SceneDev S{};
   Stencil st = {{0, 0, 0}, {0.f, 0.f, 0.f}};
   if (valid) {
-    sid = scene_of(sc, ns, j);
-    S = sc[sid];
-    st = make_stencil(S, P.x[0][p], P.x[1][p], P.x[2][p]);
+    if (own) {
+      sid = sid_own;
+      S = sc[sid];
+      st = st_own;
+    } else {
+      sid = scene_of(sc, ns, j);
+      S = sc[sid];
+      st = make_stencil(S, P.x[0][p], P.x[1][p], P.x[2][p]);
+    }
   }
 
   // ---- particle inputs and stress ----------------------------------------
