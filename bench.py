@@ -57,8 +57,8 @@ KERNELS = ["count", "scan", "scatter", "p2g", "grid_update", "g2p"]
 # (x 1.000), calibrated against a 256 MiB copy, as recorded in
 # profiles/p2g_g2p_sq_counters.md (1M-particle snow workload, launch shape
 # 5,1,256,8192; 16M-particle workload, early steps, 5,0,256,8192).
-TRAFFIC_C2 = {"p2g": 188.4e6, "g2p": 220.3e6, "grid_update": 91.0e6}
-TRAFFIC_C3 = {"p2g": 2575e6, "g2p": 3069e6}
+TRAFFIC_C2 = {"p2g": 184.2e6, "g2p": 220.5e6, "grid_update": 90.9e6}
+TRAFFIC_C3 = {"p2g": 2500e6, "g2p": 3068e6}
 # VALU instructions per wave (one wave = 64 particles) from rocprofv3
 # SQ_INSTS_VALU / SQ_WAVES, recorded in profiles/p2g_g2p_sq_counters.md; P2G
 # by its lane assignment (lanes keep their particles, layers, pairs).  The
