@@ -1141,20 +1141,36 @@ __global__ __launch_bounds__(kG2PMaxWG) void k_g2p(PBuf cur, PBuf nxt, const Sce
   // Tile node li of every staged tile; outside the grid (v, m) = 0.  A node
   // with m < 1e-12, which every gather skips, is staged with v = 0, so the
   // FP32 gather's product with it is zero without a test.
+  // The node loads of four tiles are issued before any is waited for (one
+  // round trip per tile otherwise lengthens every workgroup's start).
   for (int li = t; li < kTileNodes; li += (int)blockDim.x) {
     const int lx = li % kTileSide, ly = (li / kTileSide) % kTileSide,
               lz = li / (kTileSide * kTileSide);
-    for (int ti = 0; ti < ntiles; ti++) {
-      const int x = gmeta[ti][0] + lx, y = gmeta[ti][1] + ly, z = gmeta[ti][2] + lz;
-      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (x < gmeta[ti][3] && y < gmeta[ti][4] && z < gmeta[ti][5]) {
-        uint32_t ux, uy, uz;
-        node_axes(gboff[ti], gmeta[ti][6], gmeta[ti][7], x, y, z, ux, uy, uz);
-        v = G.gv[ux + uy + uz];
-        if (v.w < 1e-12f) v.x = v.y = v.z = 0.0f;
-        if (!(isfinite(v.x) && isfinite(v.y) && isfinite(v.z))) gnonfinite = 1;
+    for (int t0 = 0; t0 < ntiles; t0 += 4) {
+      float4 v[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int ti = t0 + q;
+        v[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (ti < ntiles) {
+          const int x = gmeta[ti][0] + lx, y = gmeta[ti][1] + ly, z = gmeta[ti][2] + lz;
+          if (x < gmeta[ti][3] && y < gmeta[ti][4] && z < gmeta[ti][5]) {
+            uint32_t ux, uy, uz;
+            node_axes(gboff[ti], gmeta[ti][6], gmeta[ti][7], x, y, z, ux, uy, uz);
+            v[q] = G.gv[ux + uy + uz];
+          }
+        }
       }
-      gtile[ti * kTileNodes + li] = v;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int ti = t0 + q;
+        if (ti < ntiles) {
+          float4 w = v[q];  // (0, 0, 0, 0) outside the grid
+          if (w.w < 1e-12f) w.x = w.y = w.z = 0.0f;
+          if (!(isfinite(w.x) && isfinite(w.y) && isfinite(w.z))) gnonfinite = 1;
+          gtile[ti * kTileNodes + li] = w;
+        }
+      }
     }
   }
   __syncthreads();
